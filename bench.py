@@ -1,0 +1,161 @@
+"""Benchmark: cone-beam FDK, 512x512x256 volume from 1080 views of a
+1024 x 256 flat detector (FP32 compute), on one MI355X.
+
+One step = one pass of the FDK hot path over the resident sinogram:
+restore the raw sinogram (device copy), cosine weighting, ramp filter
+(FP32 direct or MFMA), repack into the back-projector's tap layout,
+pixel-driven back-projection.  value = GUPS = voxel-view updates of the step
+(512*512*256*1080) / step time.  The roofline object is the back-projection
+kernel's, timed with HIP events on the stream it runs on; its algorithmic
+bytes per update are 4 taps x 4 B (FP32 storage; 2 B for FP16/BF16) plus
+4 B per voxel written.  cpu_baseline: the oracle's C/OpenMP restatement of
+the same back-projection on a slab of slices (all views, all columns),
+timed on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c1_config(storage: str, filter_method: str) -> dict:
+    return {
+        "InputDir": "/nonexistent", "OutputDir": os.environ.get("TMPDIR", "/tmp") + "/mct_bench",
+        "InputFiles": "x", "OutputFilePrefix": "", "OutputFileReplace": ["a", "b"],
+        "DetectorElementCountHorizontal": 1024, "DetectorElementWidth": 0.4,
+        "DetectorOffsetHorizontal": 0.0, "DetectorElementCountVertical": 256,
+        "Views": 1080, "TotalScanAngle": 360.0,
+        "SourceIsocenterDistance": 750.0, "SourceDetectorDistance": 1250.0,
+        "ImageDimension": 512, "PixelSize": 0.3, "HammingFilter": 1.0,
+        "ConeBeam": True, "DetectorElementHeight": 0.4, "ImageDimensionZ": 256,
+        "VoxelHeight": 0.3, "DetectorOffsetVertical": 0.0, "ImageCenterZ": 0.0,
+        "SinogramStorage": storage, "RampFilterMethod": filter_method,
+    }
+
+
+def cpu_baseline(fbp, filt_host: np.ndarray, nslices: int) -> dict:
+    """Times the oracle's C back-projection on nslices central slices."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import cbp
+
+    spec = fbp.geometry_spec()
+    g = dict(img_dim=spec.img_dim, img_dim_z=spec.img_dim_z, pix=spec.pix, vox_h=spec.vox_h,
+             sid=spec.sid, sdd=spec.sdd, angles=spec.angles, u=spec.u, v=spec.v, det_h=spec.det_h,
+             cone=True, cz=spec.cz, total_scan_angle=spec.total_scan_angle, short_scan=False)
+    z0 = spec.img_dim_z // 2 - nslices // 2
+    cbp.lib()  # build/load outside the timed region
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    out = cbp.backproject_fbp(filt_host, g, z0, z0 + nslices)
+    dt = time.perf_counter() - t0
+    upd = float(spec.img_dim) ** 2 * nslices * spec.n_views
+    return {"value": upd / dt / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+            "sample": f"FP32 back-projection of {nslices} central slices x 512x512 columns x "
+                      f"1080 views ({upd:.3g} updates) in {dt:.2f} s; oracle/bp_oracle.c, OpenMP",
+            "_check": out}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--filter", default="f32", choices=["f32", "bf16", "bf16x3"])
+    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the reconstruction path runs on a single device)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from reconstruction_program_taichi_amd import phantom
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    fbp = Mgfbp(c1_config(args.storage, args.filter))
+    spec = fbp.geometry_spec()
+    nx, nz, nv = spec.img_dim, spec.img_dim_z, spec.n_views
+    ell = phantom.random_ellipsoids(30, seed=2, r_fov=70.0, z_half=30.0)
+    raw = phantom.project(ell, spec.angles, spec.u, spec.v, spec.sid, spec.sdd, device=dev)
+    work = torch.empty_like(raw)
+    vol = torch.empty((nz, nx, nx), dtype=torch.float32, device=dev)
+    plan = fbp.plan()
+    stream = torch.cuda.current_stream(dev)
+
+    bp_times = []
+    orig_bp = plan.backproject_fbp
+
+    def timed_bp(filt, dtype, layout, out):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        orig_bp(filt, dtype, layout, out)
+        e1.record(stream)
+        bp_times.append((e0, e1))
+        return out
+
+    plan.backproject_fbp = timed_bp
+
+    def step():
+        work.copy_(raw)
+        fbp.reconstruct(work, vol)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    bp_times.clear()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    bp_ms = float(np.mean([a.elapsed_time(b) for a, b in bp_times]))
+    U = float(nx) * nx * nz * nv
+    gups = U / (ms_step * 1e-3) / 1e9
+    tap_bytes = 4 if args.storage == "f32" else 2
+    bytes_alg = U * 4 * tap_bytes + float(nx) * nx * nz * 4
+    achieved = bytes_alg / (bp_ms * 1e-3) / 1e9
+    line = {
+        "metric": "FDK/backprojection GUPS", "value": round(gups, 2), "unit": "GUPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: analytic projections of 30 seeded random ellipsoids (seed 2)",
+        "config": {"workload": "cone FDK 512x512x256 volume, 1080 views, 1024x256 detector",
+                   "sinogram_storage": args.storage, "ramp_filter": args.filter,
+                   "bp_ms": round(bp_ms, 3), "bp_gups": round(U / (bp_ms * 1e-3) / 1e9, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "bp_cone_axial_kernel",
+                     "model": f"gather bytes: 4 taps x {tap_bytes} B per update + 4 B per voxel"},
+    }
+    if not args.no_cpu:
+        filt_host = fbp.last_filtered.cpu().numpy()
+        cb = cpu_baseline(fbp, filt_host, args.cpu_slices)
+        check = cb.pop("_check")
+        z0 = nz // 2 - args.cpu_slices // 2
+        gpu_slab = vol[z0:z0 + args.cpu_slices].cpu().numpy()
+        if args.storage == "f32":
+            denom = float(np.linalg.norm(check)) or 1.0
+            cb["rel_l2_vs_gpu"] = float(np.linalg.norm(gpu_slab - check) / denom)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

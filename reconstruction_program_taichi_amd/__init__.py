@@ -1,0 +1,26 @@
+"""MI355X-native reconstruction hot path of MandoCT-Taichi
+(SEU-CT-Recon/Reconstruction_program_taichi): sinogram weighting, ramp
+filter, pixel-driven fan/cone back-projection, bilinear ray-driven forward
+projection and the IRN-CGLS / TV / PICCS loop, as hand-written HIP kernels
+for gfx950 behind the C ABI of include/mct.h (libmct.so), driven by
+config-JSONC drop-ins for run_mgfbp / run_mgfpj / run_mgfbp_ir.
+"""
+from .config import (PI, ReadConfigFile, XuLeastSquareSolution, imreadRaw, load_jsonc,
+                     remove_comments, save_jsonc)
+
+__all__ = ["PI", "ReadConfigFile", "XuLeastSquareSolution", "imreadRaw", "load_jsonc",
+           "remove_comments", "save_jsonc", "run_mgfbp", "Mgfbp", "run_mgfpj", "Mgfpj",
+           "run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"]
+
+
+def __getattr__(name):
+    if name in ("run_mgfbp", "Mgfbp"):
+        from . import fbp
+        return getattr(fbp, name)
+    if name in ("run_mgfpj", "Mgfpj"):
+        from . import fpj
+        return getattr(fpj, name)
+    if name in ("run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"):
+        from . import ir
+        return getattr(ir, name)
+    raise AttributeError(name)

@@ -1,0 +1,578 @@
+// Pixel-driven back-projection kernels.
+//
+// Reference: Mgfbp.BackProjectionPixelDriven (run_mgfbp.py:1062-1187) and
+// Mgfbp_ir.BackProjectionPixelDrivenPerView + SetTruncatedRegionToZero
+// (run_mgfbp_iterative_recon_IRN_CGLS.py:372-481).
+//
+// Axial fast path (no P-matrix): one thread owns an (x, y) column and ZC
+// consecutive slices of it.  Everything that depends only on (column, view)
+// - the source distance L, the magnification, the detector u index and the
+// view weight - is computed once per view and reused for the ZC slices, so a
+// voxel-view update costs one v index, one vector load of the taps and three
+// FMAs.  The filtered sinogram arrives pre-packed so that the taps of one
+// update are a single 16-byte (FP32) or 8-byte (FP16/BF16) load: QUAD layout
+// [view][row-1][col-1] for cone beam, PAIR layout [row][view][col-1] for fan
+// beam.  Workgroups are remapped so that each XCD owns a contiguous range of
+// slice chunks: its L2 then only holds the detector rows that range projects
+// onto.
+//
+// Mask semantics (run_mgfbp.py:1142-1144, 1170-1173): a voxel whose
+// projection leaves the detector in any view is 0.  The decisions follow the
+// reference's FP32 expression order (the library is built with
+// -ffp-contract=off and IEEE division): the u test once per column and view;
+// the v test through a reciprocal multiply whose sign always agrees with the
+// reference, redone with the reference's divisions within 1e-3 of the upper
+// edge.  The interpolation itself uses explicit FMAs.
+#include "mct_internal.h"
+
+namespace {
+
+struct BpArgs {
+    int nx, nz, nu, nv, nviews;
+    float pix, vh, cx, cy, cz, cos_rot, sin_rot;
+    float sid, sdd, div_factor, det_h;
+    float u0, du, v0, dv, adv, vinv;  // vinv = adv / (det_h * dv)
+    int curved, cone, view_mode, apply_pm;
+    const float *pm, *center_z;
+    const float4 *vt;  // {cos, sin, dbeta, 0} of (angle - img_rot) per view
+    const void *sino;
+    const float *mask;  // IR only: ray mask multiplying the sinogram (or NULL)
+    int layout;
+    float *vol;
+    int tiles_x, tiles_y, nzc, nxcd;
+};
+
+template <int DT> struct Taps;
+template <> struct Taps<MCT_F32> {
+    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
+        return reinterpret_cast<const float4 *>(base)[i];
+    }
+    __device__ __forceinline__ static float2 pair(const void *base, size_t i) {
+        return reinterpret_cast<const float2 *>(base)[i];
+    }
+    __device__ __forceinline__ static float one(const void *base, size_t i) {
+        return reinterpret_cast<const float *>(base)[i];
+    }
+};
+template <> struct Taps<MCT_F16> {
+    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
+        ushort4 h = reinterpret_cast<const ushort4 *>(base)[i];
+        return make_float4(mct::f16_to_f32(h.x), mct::f16_to_f32(h.y), mct::f16_to_f32(h.z),
+                           mct::f16_to_f32(h.w));
+    }
+    __device__ __forceinline__ static float2 pair(const void *base, size_t i) {
+        ushort2 h = reinterpret_cast<const ushort2 *>(base)[i];
+        return make_float2(mct::f16_to_f32(h.x), mct::f16_to_f32(h.y));
+    }
+    __device__ __forceinline__ static float one(const void *base, size_t i) {
+        return mct::f16_to_f32(reinterpret_cast<const uint16_t *>(base)[i]);
+    }
+};
+template <> struct Taps<MCT_BF16> {
+    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
+        uint2 w = reinterpret_cast<const uint2 *>(base)[i];
+        return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u),
+                           __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xffff0000u));
+    }
+    __device__ __forceinline__ static float2 pair(const void *base, size_t i) {
+        uint32_t w = reinterpret_cast<const uint32_t *>(base)[i];
+        return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+    }
+    __device__ __forceinline__ static float one(const void *base, size_t i) {
+        return mct::bf16_to_f32(reinterpret_cast<const uint16_t *>(base)[i]);
+    }
+};
+
+// Logical tile of this workgroup: 16x16 columns and one chunk of ZC slices.
+__device__ __forceinline__ void decode_tile(const BpArgs &a, int &ix, int &iy, int &zc_idx) {
+    int nb = a.tiles_x * a.tiles_y * a.nzc;
+    int b = mct::xcd_remap((int)blockIdx.x, nb, a.nxcd);
+    int t = b % (a.tiles_x * a.tiles_y);
+    zc_idx = b / (a.tiles_x * a.tiles_y);
+    ix = (t % a.tiles_x) * 16 + (int)(threadIdx.x & 15);
+    iy = (t / a.tiles_x) * 16 + (int)(threadIdx.x >> 4);
+}
+
+// In-plane position of voxel column (ix, iy) after ImageRotation (axial
+// view), run_mgfbp.py:1089-1091, 1103-1104.
+__device__ __forceinline__ void voxel_xy(const BpArgs &a, int ix, int iy, float &x, float &y) {
+    float half = (float)(a.nx - 1) / 2.0f;
+    float xa = a.pix * ((float)ix - half) + a.cx;
+    float ya = (-a.pix) * ((float)iy - half) + a.cy;
+    x = xa * a.cos_rot + ya * a.sin_rot;
+    y = (-xa) * a.sin_rot + ya * a.cos_rot;
+}
+
+struct ColumnView {
+    bool ok;
+    int uf;
+    float ru, mag, coef;
+};
+
+// Per (column, view) geometry of run_mgfbp.py:1122-1155.
+__device__ __forceinline__ ColumnView column_view(const BpArgs &a, float x, float y, float4 t) {
+    ColumnView cv;
+    float L = a.sid - x * t.x - y * t.y;
+    cv.mag = a.sdd / L;
+    float yr = (-x) * t.y + y * t.x;
+    float u = a.curved ? a.sdd * atan2f(yr, L) : cv.mag * yr;
+    float uidx = (u - a.u0) / a.du;
+    cv.ok = !(uidx < 0.f || uidx + 1.f > (float)(a.nu - 1));
+    float uff = floorf(uidx);
+    int uf = (int)uff;
+    uf = uf < 0 ? 0 : (uf > a.nu - 2 ? a.nu - 2 : uf);
+    cv.uf = uf;
+    cv.ru = uidx - (float)uf;
+    float w;
+    if (a.curved) {
+        float q = x * t.y - y * t.x;
+        w = 1.0f / ((L * L) + q * q);
+    } else {
+        w = 1.0f / (L * L);
+    }
+    cv.coef = (a.sid * w) * t.z * a.div_factor;
+    return cv;
+}
+
+// Detector row index of run_mgfbp.py:1157-1164 and its range test
+// (run_mgfbp.py:1170-1173).  Returns false when the voxel is masked.
+__device__ __forceinline__ bool v_index(const BpArgs &a, float vproj, float vmax, int &vf,
+                                        float &rv) {
+    float d = vproj - a.v0;
+    float t = d * a.vinv;
+    if (t < 0.f) return false;  // same sign as the reference's quotient
+    if (t > vmax - 1e-3f) {
+        float te = d / a.det_h * a.adv / a.dv;
+        if (floorf(te) + 1.f > vmax) return false;
+    }
+    int f = (int)t;
+    f = f > a.nv - 2 ? a.nv - 2 : f;
+    vf = f;
+    rv = t - (float)f;
+    return true;
+}
+
+// Cone beam, axial view, QUAD taps; ZC slices per thread.
+template <int ZC, int DT>
+__global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
+    int ix, iy, zci;
+    decode_tile(a, ix, iy, zci);
+    if (ix >= a.nx || iy >= a.nx) return;
+    const int z0 = zci * ZC;
+    float x, y;
+    voxel_xy(a, ix, iy, x, y);
+    const float halfz = (float)(a.nz - 1) / 2.0f;
+    float zs[ZC], acc[ZC];
+#pragma unroll
+    for (int k = 0; k < ZC; ++k) {
+        zs[k] = ((float)(z0 + k) - halfz) * a.vh + a.cz;
+        acc[k] = 0.f;
+    }
+    const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
+    uint32_t alive = nk >= 32 ? 0xffffffffu : ((1u << nk) - 1u);
+    const float vmax = (float)(a.nv - 1);
+    const int ncol = a.nu - 1;
+    const size_t view_stride = (size_t)(a.nv - 1) * (size_t)ncol;
+    for (int j = 0; j < a.nviews && alive; ++j) {
+        const float4 t = a.vt[j];
+        ColumnView cv = column_view(a, x, y, t);
+        if (!cv.ok) {
+            alive = 0;
+            break;
+        }
+        const size_t vbase = (size_t)j * view_stride + (size_t)cv.uf;
+#pragma unroll
+        for (int k = 0; k < ZC; ++k) {
+            if (!(alive & (1u << k))) continue;
+            int vf;
+            float rv;
+            if (!v_index(a, cv.mag * zs[k], vmax, vf, rv)) {
+                alive &= ~(1u << k);
+                continue;
+            }
+            float4 q = Taps<DT>::quad(a.sino, vbase + (size_t)vf * ncol);
+            float p0 = fmaf(cv.ru, q.y - q.x, q.x);
+            float p1 = fmaf(cv.ru, q.w - q.z, q.z);
+            acc[k] = fmaf(cv.coef, fmaf(rv, p1 - p0, p0), acc[k]);
+        }
+    }
+    const size_t plane = (size_t)a.nx * a.nx;
+    float *out = a.vol + (size_t)iy * a.nx + ix;
+#pragma unroll
+    for (int k = 0; k < ZC; ++k)
+        if (k < nk) out[(size_t)(z0 + k) * plane] = (alive & (1u << k)) ? acc[k] : 0.f;
+}
+
+// Fan beam, axial view, PAIR taps; slice z reads detector row z.
+template <int ZC, int DT>
+__global__ __launch_bounds__(256) void bp_fan_axial_kernel(BpArgs a) {
+    int ix, iy, zci;
+    decode_tile(a, ix, iy, zci);
+    if (ix >= a.nx || iy >= a.nx) return;
+    const int z0 = zci * ZC;
+    const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
+    float x, y;
+    voxel_xy(a, ix, iy, x, y);
+    float acc[ZC];
+#pragma unroll
+    for (int k = 0; k < ZC; ++k) acc[k] = 0.f;
+    bool alive = true;
+    const int ncol = a.nu - 1;
+    const size_t row_stride = (size_t)a.nviews * ncol;
+    for (int j = 0; j < a.nviews; ++j) {
+        const float4 t = a.vt[j];
+        ColumnView cv = column_view(a, x, y, t);
+        if (!cv.ok) {
+            alive = false;
+            break;
+        }
+        const size_t base = (size_t)z0 * row_stride + (size_t)j * ncol + cv.uf;
+#pragma unroll
+        for (int k = 0; k < ZC; ++k) {
+            if (k < nk) {
+                float2 p = Taps<DT>::pair(a.sino, base + (size_t)k * row_stride);
+                acc[k] = fmaf(cv.coef, fmaf(cv.ru, p.y - p.x, p.x), acc[k]);
+            }
+        }
+    }
+    const size_t plane = (size_t)a.nx * a.nx;
+    float *out = a.vol + (size_t)iy * a.nx + ix;
+#pragma unroll
+    for (int k = 0; k < ZC; ++k)
+        if (k < nk) out[(size_t)(z0 + k) * plane] = alive ? acc[k] : 0.f;
+}
+
+// Voxel position for any ReconViewMode, run_mgfbp.py:1086-1104.
+__device__ __forceinline__ void voxel_xyz(const BpArgs &a, int ix, int iy, int iz, float czv,
+                                          float &x, float &y, float &z) {
+    float half = (float)(a.nx - 1) / 2.0f;
+    float halfz = (float)(a.nz - 1) / 2.0f;
+    float xa, ya;
+    if (a.view_mode == 1) {
+        xa = a.pix * ((float)ix - half) + a.cx;
+        ya = (-a.pix) * ((float)iy - half) + a.cy;
+        z = ((float)iz - halfz) * a.vh + czv;
+    } else if (a.view_mode == 2) {
+        xa = a.pix * ((float)ix - half) + a.cx;
+        z = (-a.pix) * ((float)iy - half) + czv;
+        ya = (-((float)iz - halfz)) * a.vh + a.cy;
+    } else {
+        z = (-a.pix) * ((float)iy - half) + czv;
+        ya = (-a.pix) * ((float)ix - half) + a.cy;
+        xa = ((float)iz - halfz) * a.vh + a.cx;
+    }
+    x = xa * a.cos_rot + ya * a.sin_rot;
+    y = (-xa) * a.sin_rot + ya * a.cos_rot;
+}
+
+// Generic FBP back-projection: one thread per voxel; every view mode and the
+// P-matrix branch (run_mgfbp.py:1133-1141, 1165-1168), QUAD/PAIR/PLAIN taps.
+template <int DT>
+__global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
+    int64_t n = (int64_t)a.nz * a.nx * a.nx;
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    int ix = (int)(id % a.nx);
+    int iy = (int)((id / a.nx) % a.nx);
+    int iz = (int)(id / ((int64_t)a.nx * a.nx));
+    float x, y, z;
+    voxel_xyz(a, ix, iy, iz, a.cz, x, y, z);
+    const int ncol = a.nu - 1;
+    float acc = 0.f;
+    bool alive = true;
+    for (int j = 0; j < a.nviews; ++j) {
+        const float4 t = a.vt[j];
+        float L = a.sid - x * t.x - y * t.y;
+        float mag, uidx;
+        if (!a.apply_pm) {
+            mag = a.sdd / L;
+            float yr = (-x) * t.y + y * t.x;
+            float u = a.curved ? a.sdd * atan2f(yr, L) : mag * yr;
+            uidx = (u - a.u0) / a.du;
+        } else {
+            const float *P = a.pm + 12 * j;
+            mag = 1.0f / (P[8] * x + P[9] * y + P[10] * z + P[11] * 1.0f);
+            uidx = (P[0] * x + P[1] * y + P[2] * z + P[3] * 1.0f) * mag;
+        }
+        if (uidx < 0.f || uidx + 1.f > (float)(a.nu - 1)) {
+            alive = false;
+            break;
+        }
+        int uf = (int)floorf(uidx);
+        uf = uf > a.nu - 2 ? a.nu - 2 : uf;
+        float ru = uidx - (float)uf;
+        float w;
+        if (a.curved) {
+            float q = x * t.y - y * t.x;
+            w = 1.0f / ((L * L) + q * q);
+        } else {
+            w = 1.0f / (L * L);
+        }
+        float coef = (a.sid * w) * t.z * a.div_factor;
+        if (a.cone) {
+            float vidx;
+            if (!a.apply_pm) {
+                vidx = (mag * z - a.v0) / a.det_h * a.adv / a.dv;
+            } else {
+                const float *P = a.pm + 12 * j;
+                vidx = (P[4] * x + P[5] * y + P[6] * z + P[7] * 1.0f) * mag;
+            }
+            float vff = floorf(vidx);
+            if (vff < 0.f || vff + 1.f > (float)(a.nv - 1)) {
+                alive = false;
+                break;
+            }
+            int vf = (int)vff;
+            float rv = vidx - vff;
+            float p0, p1;
+            if (a.layout == MCT_QUAD) {
+                float4 q = Taps<DT>::quad(a.sino, ((size_t)j * (a.nv - 1) + vf) * ncol + uf);
+                p0 = q.x * (1.f - ru) + q.y * ru;
+                p1 = q.z * (1.f - ru) + q.w * ru;
+            } else {
+                size_t r0 = ((size_t)vf * a.nviews + j) * a.nu + uf;
+                size_t r1 = r0 + (size_t)a.nviews * a.nu;
+                p0 = Taps<DT>::one(a.sino, r0) * (1.f - ru) + Taps<DT>::one(a.sino, r0 + 1) * ru;
+                p1 = Taps<DT>::one(a.sino, r1) * (1.f - ru) + Taps<DT>::one(a.sino, r1 + 1) * ru;
+            }
+            acc += coef * ((1.f - rv) * p0 + rv * p1);
+        } else {
+            float s0, s1;
+            if (a.layout == MCT_PAIR) {
+                float2 p = Taps<DT>::pair(a.sino, ((size_t)iz * a.nviews + j) * ncol + uf);
+                s0 = p.x;
+                s1 = p.y;
+            } else {
+                size_t r0 = ((size_t)iz * a.nviews + j) * a.nu + uf;
+                s0 = Taps<DT>::one(a.sino, r0);
+                s1 = Taps<DT>::one(a.sino, r0 + 1);
+            }
+            acc += coef * ((1.f - ru) * s0 + ru * s1);
+        }
+    }
+    a.vol[id] = alive ? acc : 0.f;
+}
+
+// IR back-projection over all views (run_mgfbp_iterative_recon_IRN_CGLS.py:
+// 372-481): unfiltered, weight = ray length per voxel * pix; a voxel whose u
+// leaves the detector in any view is 0 (truncation flag); v off the detector
+// adds 0.  PLAIN FP32 sinogram [rows][views][nu]; optional mask multiplies
+// each tap.  ZC slices per thread.
+template <int ZC>
+__global__ __launch_bounds__(256) void bp_ir_axial_kernel(BpArgs a) {
+    int ix, iy, zci;
+    decode_tile(a, ix, iy, zci);
+    if (ix >= a.nx || iy >= a.nx) return;
+    const int z0 = zci * ZC;
+    const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
+    float x, y;
+    voxel_xy(a, ix, iy, x, y);
+    const float halfz = (float)(a.nz - 1) / 2.0f;
+    float zb[ZC], acc[ZC];
+#pragma unroll
+    for (int k = 0; k < ZC; ++k) {
+        zb[k] = ((float)(z0 + k) - halfz) * a.vh;
+        acc[k] = 0.f;
+    }
+    bool alive = true;
+    const float *sg = reinterpret_cast<const float *>(a.sino);
+    const size_t row_stride = (size_t)a.nviews * a.nu;
+    const float vmax = (float)(a.nv - 1);
+    for (int j = 0; j < a.nviews; ++j) {
+        const float4 t = a.vt[j];
+        const float czv = a.center_z ? a.center_z[j] : a.cz;
+        float L = a.sid - x * t.x - y * t.y;
+        float dxs = x - a.sid * t.x;
+        float dys = y - a.sid * t.y;
+        bool dflag = fabsf(dxs) > fabsf(dys);
+        float den = dflag ? fabsf(dxs) : fabsf(dys);
+        float mag = a.sdd / L;
+        float yr = (-x) * t.y + y * t.x;
+        float u = a.curved ? a.sdd * atan2f(yr, L) : mag * yr;
+        float uidx = (u - a.u0) / a.du;
+        if (uidx < 0.f || uidx + 1.f > (float)(a.nu - 1)) {
+            alive = false;
+            break;
+        }
+        int uf = (int)floorf(uidx);
+        uf = uf > a.nu - 2 ? a.nu - 2 : uf;
+        float ru = uidx - (float)uf;
+        const size_t vb = (size_t)j * a.nu + uf;
+#pragma unroll
+        for (int k = 0; k < ZC; ++k) {
+            if (k >= nk) continue;
+            float z = zb[k] + czv;
+            float wgt = sqrtf(dxs * dxs + dys * dys + z * z) / den;
+            float val;
+            if (a.cone) {
+                float vidx = (mag * z - a.v0) / a.det_h * a.adv / a.dv;
+                float vff = floorf(vidx);
+                if (vff < 0.f || vff + 1.f > vmax) continue;
+                int vf = (int)vff;
+                float rv = vidx - vff;
+                size_t r0 = (size_t)vf * row_stride + vb;
+                size_t r1 = r0 + row_stride;
+                float s00 = sg[r0], s01 = sg[r0 + 1], s10 = sg[r1], s11 = sg[r1 + 1];
+                if (a.mask) {
+                    s00 *= a.mask[r0];
+                    s01 *= a.mask[r0 + 1];
+                    s10 *= a.mask[r1];
+                    s11 *= a.mask[r1 + 1];
+                }
+                float p0 = s00 * (1.f - ru) + s01 * ru;
+                float p1 = s10 * (1.f - ru) + s11 * ru;
+                val = (1.f - rv) * p0 + rv * p1;
+            } else {
+                size_t r0 = (size_t)(z0 + k) * row_stride + vb;
+                float s0 = sg[r0], s1 = sg[r0 + 1];
+                if (a.mask) {
+                    s0 *= a.mask[r0];
+                    s1 *= a.mask[r0 + 1];
+                }
+                val = (1.f - ru) * s0 + ru * s1;
+            }
+            acc[k] += val * wgt * a.pix;
+        }
+    }
+    const size_t plane = (size_t)a.nx * a.nx;
+    float *out = a.vol + (size_t)iy * a.nx + ix;
+#pragma unroll
+    for (int k = 0; k < ZC; ++k)
+        if (k < nk) out[(size_t)(z0 + k) * plane] = alive ? acc[k] : 0.f;
+}
+
+BpArgs make_args(const mct_plan *p) {
+    const mct_geometry &g = p->g;
+    BpArgs a = {};
+    a.nx = g.img_dim;
+    a.nz = g.img_dim_z;
+    a.nu = g.n_u;
+    a.nv = g.n_v;
+    a.nviews = g.n_views;
+    a.pix = g.pix;
+    a.vh = g.vox_h;
+    a.cx = g.cx;
+    a.cy = g.cy;
+    a.cz = g.cz;
+    a.cos_rot = p->cos_rot;
+    a.sin_rot = p->sin_rot;
+    a.sid = g.sid;
+    a.sdd = g.sdd;
+    a.div_factor = p->div_factor;
+    a.det_h = g.det_h;
+    a.u0 = p->u0;
+    a.du = p->du;
+    a.v0 = p->v0;
+    a.dv = p->dv;
+    a.adv = p->adv;
+    a.vinv = (g.cone && g.det_h > 0.f) ? (float)((double)p->adv / ((double)g.det_h * (double)p->dv)) : 0.f;
+    a.curved = g.curved;
+    a.cone = g.cone;
+    a.view_mode = g.view_mode;
+    a.apply_pm = g.apply_pmatrix;
+    a.pm = p->pmatrix;
+    a.center_z = p->center_z;
+    a.vt = p->bp_view;
+    a.nxcd = p->ctx->xcd_count;
+    return a;
+}
+
+#define MCT_AXIAL_LAUNCH(KERNEL, ZC, DT) \
+    hipLaunchKernelGGL((KERNEL<ZC, DT>), dim3(nblocks), dim3(256), 0, ctx->stream, a)
+
+}  // namespace
+
+extern "C" {
+
+int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout, float *vol) {
+    MCT_CHECK(plan && filt && vol, "NULL argument");
+    const mct_geometry &g = plan->g;
+    mct_ctx *ctx = plan->ctx;
+    MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
+    if (g.cone)
+        MCT_CHECK(layout == MCT_QUAD || layout == MCT_PLAIN,
+                  "cone-beam back-projection reads QUAD or PLAIN taps (got %d)", layout);
+    else
+        MCT_CHECK(layout == MCT_PAIR || layout == MCT_PLAIN,
+                  "fan-beam back-projection reads PAIR or PLAIN taps (got %d)", layout);
+    if (!g.cone)
+        MCT_CHECK(g.n_v == g.img_dim_z,
+                  "fan beam: one detector row per slice required (rows %d, slices %d)", g.n_v,
+                  g.img_dim_z);
+    if (g.cone && !g.apply_pmatrix) MCT_CHECK(g.det_h > 0.f, "cone beam needs det_h > 0");
+    BpArgs a = make_args(plan);
+    a.sino = filt;
+    a.layout = layout;
+    a.vol = vol;
+    const bool fast = g.view_mode == 1 && !g.apply_pmatrix && layout != MCT_PLAIN;
+    if (fast) {
+        int zc = ctx->bp_zc ? ctx->bp_zc : (g.cone ? 16 : 8);
+        a.tiles_x = (g.img_dim + 15) / 16;
+        a.tiles_y = a.tiles_x;
+        a.nzc = (g.img_dim_z + zc - 1) / zc;
+        int nblocks = a.tiles_x * a.tiles_y * a.nzc;
+        if (g.cone) {
+#define CASE_DT(ZC)                                                                  \
+    if (dtype == MCT_F32) MCT_AXIAL_LAUNCH(bp_cone_axial_kernel, ZC, MCT_F32);       \
+    else if (dtype == MCT_F16) MCT_AXIAL_LAUNCH(bp_cone_axial_kernel, ZC, MCT_F16);  \
+    else MCT_AXIAL_LAUNCH(bp_cone_axial_kernel, ZC, MCT_BF16);
+            switch (zc) {
+                case 4: CASE_DT(4) break;
+                case 8: CASE_DT(8) break;
+                case 16: CASE_DT(16) break;
+                default: CASE_DT(32) break;
+            }
+#undef CASE_DT
+        } else {
+#define CASE_DT(ZC)                                                                  \
+    if (dtype == MCT_F32) MCT_AXIAL_LAUNCH(bp_fan_axial_kernel, ZC, MCT_F32);        \
+    else if (dtype == MCT_F16) MCT_AXIAL_LAUNCH(bp_fan_axial_kernel, ZC, MCT_F16);   \
+    else MCT_AXIAL_LAUNCH(bp_fan_axial_kernel, ZC, MCT_BF16);
+            switch (zc) {
+                case 4: CASE_DT(4) break;
+                case 8: CASE_DT(8) break;
+                case 16: CASE_DT(16) break;
+                default: CASE_DT(32) break;
+            }
+#undef CASE_DT
+        }
+    } else {
+        int64_t n = (int64_t)g.img_dim_z * g.img_dim * g.img_dim;
+        unsigned nblocks = mct::blocks_for(n, 256);
+        if (dtype == MCT_F32)
+            hipLaunchKernelGGL(bp_generic_kernel<MCT_F32>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
+        else if (dtype == MCT_F16)
+            hipLaunchKernelGGL(bp_generic_kernel<MCT_F16>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(bp_generic_kernel<MCT_BF16>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
+    }
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, float *vol) {
+    MCT_CHECK(plan && sgm && vol, "NULL argument");
+    const mct_geometry &g = plan->g;
+    mct_ctx *ctx = plan->ctx;
+    MCT_CHECK(g.view_mode == 1 && !g.apply_pmatrix,
+              "IR back-projection supports the axial view without P-matrix");
+    if (!g.cone)
+        MCT_CHECK(g.n_v == g.img_dim_z, "fan beam: one detector row per slice required");
+    else
+        MCT_CHECK(g.det_h > 0.f, "cone beam needs det_h > 0");
+    BpArgs a = make_args(plan);
+    a.sino = sgm;
+    a.mask = ray_mask;
+    a.layout = MCT_PLAIN;
+    a.vol = vol;
+    const int zc = 8;
+    a.tiles_x = (g.img_dim + 15) / 16;
+    a.tiles_y = a.tiles_x;
+    a.nzc = (g.img_dim_z + zc - 1) / zc;
+    int nblocks = a.tiles_x * a.tiles_y * a.nzc;
+    hipLaunchKernelGGL(bp_ir_axial_kernel<8>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+}  // extern "C"

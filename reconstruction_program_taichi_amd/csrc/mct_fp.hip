@@ -1,0 +1,392 @@
+// Ray-driven forward projection, ray mask and Poisson noise.
+//
+// Reference: Mgfpj.ForwardProjectionBilinear + BinSinogram
+// (run_mgfpj.py:463-616), Mgfpj_v3.ForwardProjectionBilinear
+// (run_mgfpj_v3.py:184-366), Mgfbp_ir.ForwardProjectionBilinear and
+// GenEffectiveMapForwardProjectionAgent
+// (run_mgfbp_iterative_recon_IRN_CGLS.py:494-833), Mgfpj.AddPossionNoise
+// (run_mgfpj.py:643-654).
+//
+// The gantry is fixed and the object rotates: the source sits at
+// (sid, 0, z_s), detector element (u, v) at (-(sdd - sid), u, v) (flat) and
+// the ray is sampled at l = l_min + k*step*diag for k < count, each sample
+// rotated by the view angle, tested against the volume (index and index+1
+// inside) and interpolated tri/bilinearly; the sum of samples times
+// step*diag is the line integral.  One thread traces one ray for every
+// detector row and view in a single launch; the loop over k starts and ends
+// at the ray's intersection with the volume's bounding box (FP64, widened by
+// one step on each side), which skips samples the in-bounds test would
+// reject anyway, so the sum is unchanged.
+#include "mct_internal.h"
+
+#include <cmath>
+
+namespace {
+
+struct FpArgs {
+    int nx, nz, nu_os, nu_out, nrows, nviews, os, count;
+    float pix, inv_pix, vh, inv_vh, x0, y0, zbase, cz;
+    float sid, sdd, lmin, step, diag, zpv;
+    int cone, curved;
+    const float *u, *v, *center_z;
+    const float2 *ft;  // {cos, sin} of the view angle
+    const float *vol;
+    const float *mask;  // NULL or [nrows][nviews][nu_out]
+    float *out;         // [nrows][nviews][nu_out]
+};
+
+// Source and unit direction of ray (u index ui, row vi, view j):
+// run_mgfpj.py:531-551.
+__device__ __forceinline__ void ray_setup(const FpArgs &a, int ui, int vi, int j, float &sz,
+                                          float &ux, float &uy, float &uz) {
+    float dx, dy;
+    if (a.curved) {
+        float gp = (-a.u[ui]) / a.sdd;
+        dx = -a.sdd * cosf(gp) + a.sid;
+        dy = -a.sdd * sinf(gp);
+    } else {
+        dx = -(a.sdd - a.sid);
+        dy = a.u[ui];
+    }
+    float zshift = a.zpv * (float)j;
+    float dz = a.v[vi] + zshift;
+    sz = zshift;
+    float ex = dx - a.sid, ey = dy - 0.0f, ez = dz - sz;
+    float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+    ux = ex / dist;
+    uy = ey / dist;
+    uz = ez / dist;
+}
+
+// Range [k_lo, k_hi) of sample indexes that can fall inside the volume's
+// bounding box (plus one step of margin on each side).
+__device__ __forceinline__ void clip_range(const FpArgs &a, float sz, float ux, float uy, float uz,
+                                           float c, float s, float z0, int &k_lo, int &k_hi) {
+    // rotated source and direction (FP64)
+    double px = (double)a.sid * c - 0.0 * s, py = 0.0 * c + (double)a.sid * s, pz = sz;
+    double dx = (double)ux * c - (double)uy * s, dy = (double)uy * c + (double)ux * s, dz = uz;
+    double lo = -1e300, hi = 1e300;
+    // box in rotated coordinates: x in [x0, x0+(nx-1)p], y in [y0-(nx-1)p, y0]
+    double bx0 = a.x0, bx1 = (double)a.x0 + (double)(a.nx - 1) * a.pix;
+    double by0 = (double)a.y0 - (double)(a.nx - 1) * a.pix, by1 = a.y0;
+    auto slab = [&](double p, double d, double b0, double b1) {
+        if (fabs(d) < 1e-300) {
+            if (p < b0 || p > b1) { lo = 1e300; hi = -1e300; }
+            return;
+        }
+        double t0 = (b0 - p) / d, t1 = (b1 - p) / d;
+        if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
+        lo = t0 > lo ? t0 : lo;
+        hi = t1 < hi ? t1 : hi;
+    };
+    slab(px, dx, bx0, bx1);
+    slab(py, dy, by0, by1);
+    if (a.cone) slab(pz, dz, (double)z0, (double)z0 + (double)(a.nz - 1) * a.vh);
+    if (!(lo <= hi)) {
+        k_lo = k_hi = 0;
+        return;
+    }
+    double dl = (double)a.step * (double)a.diag;
+    double klo = floor((lo - a.lmin) / dl) - 1.0, khi = ceil((hi - a.lmin) / dl) + 2.0;
+    klo = klo < 0.0 ? 0.0 : klo;
+    khi = khi > (double)a.count ? (double)a.count : khi;
+    k_lo = (int)klo;
+    k_hi = (int)khi;
+    if (k_hi < k_lo) k_hi = k_lo;
+}
+
+// Line integral of one ray: run_mgfpj.py:554-605.
+__device__ float trace(const FpArgs &a, int ui, int vi, int j) {
+    float sz, ux, uy, uz;
+    ray_setup(a, ui, vi, j, sz, ux, uy, uz);
+    const float2 cs = a.ft[j];
+    const float c = cs.x, s = cs.y;
+    const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
+    int k_lo, k_hi;
+    clip_range(a, sz, ux, uy, uz, c, s, z0, k_lo, k_hi);
+    // rotated ray: P(l) = R (S + l U)
+    const float ax = a.sid * c, ay = a.sid * s;
+    const float bx = ux * c - uy * s, by = uy * c + ux * s;
+    const float dl = a.step * a.diag;
+    const int nx = a.nx;
+    const size_t plane = (size_t)nx * nx;
+    float sum = 0.f;
+    for (int k = k_lo; k < k_hi; ++k) {
+        float l = (float)k * a.step * a.diag + a.lmin;
+        float xr = fmaf(l, bx, ax);
+        float yr = fmaf(l, by, ay);
+        float fx = (xr - a.x0) * a.inv_pix;
+        float fy = (yr - a.y0) * (-a.inv_pix);
+        float flx = floorf(fx), fly = floorf(fy);
+        int xi = (int)flx, yi = (int)fly;
+        if (xi < 0 || xi + 1 >= nx || yi < 0 || yi + 1 >= nx) continue;
+        float xw = fx - flx, yw = fy - fly;
+        if (a.cone) {
+            float zr = fmaf(l, uz, sz);
+            float fz = (zr - z0) * a.inv_vh;
+            float flz = floorf(fz);
+            int zi = (int)flz;
+            if (zi < 0 || zi + 1 >= a.nz) continue;
+            float zw = fz - flz;
+            const float *p = a.vol + (size_t)zi * plane + (size_t)yi * nx + xi;
+            float v00 = p[0], v01 = p[1], v10 = p[nx], v11 = p[nx + 1];
+            const float *q = p + plane;
+            float w00 = q[0], w01 = q[1], w10 = q[nx], w11 = q[nx + 1];
+            float lo0 = fmaf(xw, v01 - v00, v00), lo1 = fmaf(xw, v11 - v10, v10);
+            float hi0 = fmaf(xw, w01 - w00, w00), hi1 = fmaf(xw, w11 - w10, w10);
+            float lo = fmaf(yw, lo1 - lo0, lo0), hi = fmaf(yw, hi1 - hi0, hi0);
+            sum += fmaf(zw, hi - lo, lo);
+        } else {
+            const float *p = a.vol + (size_t)vi * plane + (size_t)yi * nx + xi;
+            float v00 = p[0], v01 = p[1], v10 = p[nx], v11 = p[nx + 1];
+            float lo0 = fmaf(xw, v01 - v00, v00), lo1 = fmaf(xw, v11 - v10, v10);
+            sum += fmaf(yw, lo1 - lo0, lo0);
+        }
+    }
+    return sum * dl;
+}
+
+// One thread per output element (u_out, row, view): traces its os sub-rays
+// and bins them (run_mgfpj.py:609-616).
+__global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
+    int uo = blockIdx.x * 64 + (threadIdx.x & 63);
+    int vi = blockIdx.y * 4 + (threadIdx.x >> 6);
+    int j = blockIdx.z;
+    if (uo >= a.nu_out || vi >= a.nrows) return;
+    size_t oidx = ((size_t)vi * a.nviews + j) * a.nu_out + uo;
+    if (a.mask && a.mask[oidx] != 1.0f) {
+        a.out[oidx] = 0.f;
+        return;
+    }
+    float acc = 0.f;
+    for (int i = 0; i < a.os; ++i) acc += trace(a, uo * a.os + i, vi, j);
+    a.out[oidx] = acc / (float)a.os;
+}
+
+// Mgfbp_ir.GenEffectiveMapForwardProjectionAgent (cone beam): 0 when a
+// sample inside the image and the inscribed disc leaves the z range.
+__global__ __launch_bounds__(256) void ray_mask_kernel(FpArgs a) {
+    int ui = blockIdx.x * 64 + (threadIdx.x & 63);
+    int vi = blockIdx.y * 4 + (threadIdx.x >> 6);
+    int j = blockIdx.z;
+    if (ui >= a.nu_out || vi >= a.nrows) return;
+    size_t oidx = ((size_t)vi * a.nviews + j) * a.nu_out + ui;
+    float sz, ux, uy, uz;
+    ray_setup(a, ui, vi, j, sz, ux, uy, uz);
+    const float2 cs = a.ft[j];
+    const float c = cs.x, s = cs.y;
+    const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
+    int k_lo, k_hi;
+    FpArgs b = a;
+    b.cone = 0;  // clip in-plane only: the test fires when z leaves the range
+    clip_range(b, sz, ux, uy, uz, c, s, z0, k_lo, k_hi);
+    const float ax = a.sid * c, ay = a.sid * s;
+    const float bx = ux * c - uy * s, by = uy * c + ux * s;
+    const float half = (float)a.nx / 2.0f;
+    float m = 1.0f;
+    for (int k = k_lo; k < k_hi; ++k) {
+        float l = (float)k * a.step * a.diag + a.lmin;
+        float xr = fmaf(l, bx, ax);
+        float yr = fmaf(l, by, ay);
+        int xi = (int)floorf((xr - a.x0) * a.inv_pix);
+        int yi = (int)floorf((yr - a.y0) * (-a.inv_pix));
+        if (xi < 0 || xi + 1 >= a.nx || yi < 0 || yi + 1 >= a.nx) continue;
+        float dxi = (float)xi - half, dyi = (float)yi - half;
+        if (dxi * dxi + dyi * dyi < half * half) {
+            float zr = fmaf(l, uz, sz);
+            int zi = (int)floorf((zr - z0) * a.inv_vh);
+            if (zi < 0 || zi + 1 >= a.nz) {
+                m = 0.f;
+                break;
+            }
+        }
+    }
+    a.out[oidx] = m;
+}
+
+__global__ void fill_kernel(float *p, int64_t n, float v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) + Box-Muller ----
+__device__ __forceinline__ void philox_round(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
+                                             uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+}
+
+__device__ __forceinline__ uint4 philox4x32_10(uint64_t ctr, uint64_t key) {
+    uint32_t c0 = (uint32_t)ctr, c1 = (uint32_t)(ctr >> 32), c2 = 0, c3 = 0;
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+    for (int r = 0; r < 10; ++r) {
+        philox_round(c0, c1, c2, c3, k0, k1);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// Standard normal for element i: Box-Muller on the first two words of
+// Philox(counter = i, key = seed), in FP64.
+__device__ __forceinline__ double normal_of(uint64_t i, uint64_t seed) {
+    uint4 r = philox4x32_10(i, seed);
+    double u1 = ((double)r.x + 1.0) * (1.0 / 4294967296.0);  // (0, 1]
+    double u2 = (double)r.y * (1.0 / 4294967296.0);          // [0, 1)
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+}
+
+// run_mgfpj.py:643-654.
+__global__ void noise_kernel(float *sgm, int64_t n, float photons, uint64_t seed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = photons * expf(-sgm[i]);
+    t = t + (float)normal_of((uint64_t)i, seed) * sqrtf(t);
+    if (t <= 0.f) t = 1e-6f;
+    sgm[i] = logf(photons / t);
+}
+
+// Host side: scalars of run_mgfpj.py:486-527 in FP32, in the reference's order.
+struct FpScalars {
+    float x0, y0, zbase, lmin, diag;
+    int count;
+};
+
+FpScalars fp_scalars(const mct_geometry &g, int count_mode) {
+    FpScalars r;
+    float nx = (float)g.img_dim, nz = (float)g.img_dim_z;
+    float img_dimension = (float)g.img_dim * g.pix;
+    float img_dimension_z = (float)g.img_dim_z * g.vox_h;
+    r.x0 = -(nx - 1.0f) / 2.0f * g.pix + g.cx;
+    r.y0 = -(nx - 1.0f) / 2.0f * (-g.pix) + g.cy;
+    r.zbase = -(nz - 1.0f) / 2.0f * g.vox_h;
+    float D = std::sqrt(2.0f * (img_dimension * img_dimension) + img_dimension_z * img_dimension_z);
+    r.lmin = g.sid - D / 2.0f;
+    float lmax = g.sid + D / 2.0f;
+    r.diag = std::sqrt(2.0f * (g.pix * g.pix) + g.vox_h * g.vox_h);
+    float span = count_mode == 1 ? D : (lmax - r.lmin);
+    r.count = (int)std::floor(span / (g.fpj_step * r.diag));
+    return r;
+}
+
+}  // namespace
+
+namespace mct {
+
+// Distance the gantry moves per view for a helical scan (run_mgfpj.py:518-523).
+float helical_z_per_view(const mct_plan *p, const float *angles_host) {
+    const mct_geometry &g = p->g;
+    if (!g.helical) return 0.f;
+    int nv = g.n_views;
+    float tsa = std::fabs(angles_host[nv - 1] - angles_host[0]) / (float)(nv - 1) * (float)nv;
+    float laps = tsa / (kPI * 2.f);
+    return g.helical_pitch * (laps / (float)nv) * (p->adv * (float)g.n_v_full) / (g.sdd / g.sid);
+}
+
+}  // namespace mct
+
+namespace {
+
+int fill_args(mct_plan *p, FpArgs &a, int count_mode) {
+    const mct_geometry &g = p->g;
+    FpScalars s = fp_scalars(g, count_mode);
+    a = FpArgs{};
+    a.nx = g.img_dim;
+    a.nz = g.img_dim_z;
+    a.nrows = g.n_v;
+    a.nviews = g.n_views;
+    a.count = s.count;
+    a.pix = g.pix;
+    a.inv_pix = 1.0f / g.pix;
+    a.vh = g.vox_h;
+    a.inv_vh = g.vox_h > 0.f ? 1.0f / g.vox_h : 0.f;
+    a.x0 = s.x0;
+    a.y0 = s.y0;
+    a.zbase = s.zbase;
+    a.cz = g.cz;
+    a.sid = g.sid;
+    a.sdd = g.sdd;
+    a.lmin = s.lmin;
+    a.step = g.fpj_step;
+    a.diag = s.diag;
+    a.cone = g.cone;
+    a.curved = g.curved;
+    a.u = p->u;
+    a.v = p->v;
+    a.center_z = p->center_z;
+    a.ft = p->fp_view;
+    a.zpv = p->zpv;
+    return MCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mct_forward_project(mct_plan *plan, const float *vol, float *out, const float *ray_mask,
+                        int32_t oversample) {
+    MCT_CHECK(plan && vol && out, "NULL argument");
+    const mct_geometry &g = plan->g;
+    MCT_CHECK(!g.apply_pmatrix, "P-matrix forward projection is not implemented");
+    MCT_CHECK(oversample >= 1, "oversample must be >= 1 (got %d)", oversample);
+    MCT_CHECK(g.n_u % oversample == 0, "n_u (%d) is not a multiple of oversample (%d)", g.n_u,
+              oversample);
+    MCT_CHECK(!ray_mask || oversample == 1, "a ray mask requires oversample 1");
+    MCT_CHECK(g.fpj_step > 0.f, "fpj_step must be positive");
+    MCT_CHECK(!g.cone || g.vox_h > 0.f, "cone beam needs vox_h > 0");
+    MCT_CHECK(g.cone || g.n_v <= g.img_dim_z, "fan beam: row %d has no slice (%d slices)", g.n_v - 1,
+              g.img_dim_z);
+    FpArgs a;
+    fill_args(plan, a, plan->fp_count_mode);
+    a.os = oversample;
+    a.nu_os = g.n_u;
+    a.nu_out = g.n_u / oversample;
+    a.vol = vol;
+    a.mask = ray_mask;
+    a.out = out;
+    dim3 grid((a.nu_out + 63) / 64, (a.nrows + 3) / 4, a.nviews);
+    hipLaunchKernelGGL(fp_kernel, grid, dim3(256), 0, plan->ctx->stream, a);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+int mct_ray_mask(mct_plan *plan, float *mask) {
+    MCT_CHECK(plan && mask, "NULL argument");
+    const mct_geometry &g = plan->g;
+    mct_ctx *ctx = plan->ctx;
+    int64_t n = (int64_t)g.n_v * g.n_views * g.n_u;
+    if (!g.cone) {
+        hipLaunchKernelGGL(fill_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream, mask,
+                           n, 1.0f);
+        MCT_LAUNCHED();
+        return MCT_OK;
+    }
+    MCT_CHECK(!g.apply_pmatrix, "P-matrix ray mask is not implemented");
+    MCT_CHECK(g.vox_h > 0.f, "cone beam needs vox_h > 0");
+    FpArgs a;
+    fill_args(plan, a, plan->fp_count_mode);
+    a.os = 1;
+    a.nu_os = g.n_u;
+    a.nu_out = g.n_u;
+    a.out = mask;
+    dim3 grid((a.nu_out + 63) / 64, (a.nrows + 3) / 4, a.nviews);
+    hipLaunchKernelGGL(ray_mask_kernel, grid, dim3(256), 0, ctx->stream, a);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+int mct_add_noise(mct_ctx *ctx, float *sgm, int64_t n, float photon_number, uint64_t seed) {
+    MCT_CHECK(ctx && sgm, "NULL argument");
+    MCT_CHECK(photon_number > 0.f, "photon number must be positive");
+    if (n <= 0) return MCT_OK;
+    hipLaunchKernelGGL(noise_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream, sgm, n,
+                       photon_number, seed);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+}  // extern "C"

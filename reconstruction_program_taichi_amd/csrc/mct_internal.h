@@ -1,0 +1,119 @@
+// Internal declarations shared by the HIP translation units of libmct.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/mct.h"
+
+struct mct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    int xcd_count = 8;
+    int bp_zc = 0;  // 0: default slices per thread in the axial back-projector
+    // partial sums of the deterministic FP64 reductions
+    double *partials = nullptr;
+    int64_t partials_cap = 0;
+    // Toeplitz panel cache of mct_filter_rows (MFMA methods)
+    void *toep = nullptr;
+    int64_t toep_bytes = 0;
+    int toep_n = 0, toep_method = -1;
+    float toep_scale = 0.f;
+    uint64_t toep_hash = 0;
+    float *taps_dev = nullptr;  // device copy of the ramp taps (direct method)
+    int64_t taps_cap = 0;
+    float *vkern_dev = nullptr;  // device copy of the vertical Gaussian taps
+    int64_t vkern_cap = 0;
+};
+
+struct mct_plan {
+    mct_ctx *ctx = nullptr;
+    mct_geometry g{};  // scalars; the host pointers are cleared
+    // device copies of the geometry arrays
+    float *angles = nullptr, *u = nullptr, *v = nullptr;
+    float *pmatrix = nullptr, *center_z = nullptr, *matrix_a = nullptr, *x_s = nullptr;
+    // per-view tables
+    float4 *bp_view = nullptr;  // {cos(b_j - rot), sin(b_j - rot), dbeta_j, 0}
+    float2 *fp_view = nullptr;  // {cos b_j, sin b_j}
+    // host-side scalars derived once
+    float u0 = 0.f, du = 1.f;           // u[0], u[1] - u[0]
+    float v0 = 0.f, dv = 1.f, adv = 1.f;  // v[0], v[1] - v[0], |v[1] - v[0]|
+    float cos_rot = 1.f, sin_rot = 0.f;
+    float div_factor = 1.f;
+    float cos_a0 = 1.f, sin_a0 = 0.f;  // of angles[0] (P-matrix forward projection)
+    float zpv = 0.f;                   // helical z shift per view (forward projection)
+    int fp_count_mode = 0;
+};
+
+namespace mct {
+
+constexpr float kPI = 3.1415926536f;  // the reference's PI (run_mgfbp.py:16)
+
+// Records the message returned by mct_last_error() for the calling thread.
+int set_error(int code, const char *fmt, ...);
+
+#define MCT_HIP(call)                                                           \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return ::mct::set_error(MCT_ERR_DEVICE, "%s failed: %s", #call,     \
+                                    hipGetErrorString(e_));                     \
+    } while (0)
+
+#define MCT_CHECK(cond, ...)                                                    \
+    do {                                                                        \
+        if (!(cond)) return ::mct::set_error(MCT_ERR_INVALID, __VA_ARGS__);     \
+    } while (0)
+
+#define MCT_LAUNCHED()                                                          \
+    do {                                                                        \
+        hipError_t e_ = hipGetLastError();                                      \
+        if (e_ != hipSuccess)                                                   \
+            return ::mct::set_error(MCT_ERR_DEVICE, "kernel launch: %s",        \
+                                    hipGetErrorString(e_));                     \
+    } while (0)
+
+int ensure_partials(mct_ctx *ctx, int64_t n);
+
+// Grows a context-owned FP32 device buffer to at least n elements (waits for
+// the stream before freeing the old one).
+int ensure_buffer(mct_ctx *ctx, float **buf, int64_t *cap, int64_t n);
+
+// Helical z shift per view of run_mgfpj.py:518-523 (0 without a helical scan).
+float helical_z_per_view(const mct_plan *p, const float *angles_host);
+
+// div_factor of run_mgfbp.py:1072-1084 evaluated in FP32.
+float div_factor_of(const mct_geometry &g);
+
+inline unsigned blocks_for(int64_t n, int threads) {
+    return (unsigned)((n + threads - 1) / threads);
+}
+
+// Workgroups are dealt round robin over the XCDs; this maps the hardware
+// workgroup id so that XCD k receives the contiguous logical range
+// [k*per, (k+1)*per) and its L2 only sees that range's footprint.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks, int nxcd) {
+    if (nxcd <= 1 || nblocks % nxcd != 0) return bid;
+    int per = nblocks / nxcd;
+    return (bid % nxcd) * per + bid / nxcd;
+}
+
+// Round-to-nearest-even FP32 -> BF16 / FP16 bit patterns and back.
+__device__ __forceinline__ uint16_t f32_to_bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ uint16_t f32_to_f16_rn(float f) {
+    _Float16 h = (_Float16)f;
+    return __builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ float f16_to_f32(uint16_t h) {
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+
+}  // namespace mct

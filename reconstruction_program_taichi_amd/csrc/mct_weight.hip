@@ -1,0 +1,248 @@
+// Sinogram weighting (BH polynomial, cosine, Parker), vertical Gaussian and
+// the repack into the back-projector's tap layouts.
+#include "mct_internal.h"
+
+namespace {
+
+using mct::kPI;
+constexpr int kMaxBH = 16;
+
+struct BHCoef {
+    float c[kMaxBH];
+};
+
+// One thread per (row s, view i, element j) of a PLAIN [n_v][n_views][n_u]
+// sinogram.  BHCorrection (run_mgfbp.py:946-954) then WeightSgm
+// (run_mgfbp.py:956-1011) operation by operation in FP32; the library is
+// built with -ffp-contract=off, so no product is fused into an FMA here.
+__global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int nu,
+                              const float *__restrict__ uarr, const float *__restrict__ varr,
+                              const float *__restrict__ ang, float sdd, float scan,
+                              int short_scan, int curved, BHCoef bh, int bh_order) {
+    int64_t n = (int64_t)nv * nviews * nu;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    int j = (int)(idx % nu);
+    int i = (int)((idx / nu) % nviews);
+    int s = (int)(idx / ((int64_t)nu * nviews));
+    float p = sgm[idx];
+    if (bh_order > 0) {
+        float acc = 0.f, pw = p;
+        for (int t = 0; t < bh_order; ++t) {
+            acc = acc + bh.c[t] * pw;
+            pw = pw * p;
+        }
+        p = acc;
+    }
+    float u = uarr[j];
+    float v = varr[s];
+    if (curved) {
+        p = p * sdd * cosf((-1.f) * u / sdd) * sdd / sqrtf(sdd * sdd + v * v);
+    } else {
+        p = (p * sdd * sdd) / sqrtf(sdd * sdd + u * u + v * v);
+    }
+    if (short_scan) {
+        float ascan = fabsf(scan);
+        float num_rounds = floorf(ascan / (kPI * 2.f));
+        float remain = ascan - num_rounds * kPI * 2.f;
+        float beta = fabsf(ang[i] - ang[0]);
+        float dir = ascan / scan;
+        float gamma = curved ? ((-1.f) * u / sdd) * dir : atan2f((-1.f) * u, sdd) * dir;
+        float gmax = remain - kPI;
+        float w = 0.f;
+        if (remain <= kPI && num_rounds == 0.f) {
+            w = 1.f;
+        } else if (remain <= kPI && num_rounds >= 1.f) {
+            if (0.f <= beta && beta < remain)
+                w = beta / remain;
+            else if (remain <= beta && beta < 2.f * kPI * num_rounds)
+                w = 1.f;
+            else if (2.f * kPI * num_rounds <= beta && beta <= (2.f * kPI * num_rounds + remain))
+                w = (2.f * kPI * num_rounds + remain - beta) / remain;
+        } else if (kPI < remain && remain <= 2.f * kPI) {
+            if (0.f <= beta && beta < (gmax - 2.f * gamma)) {
+                w = sinf(kPI / 2.f * beta / (gmax - 2.f * gamma));
+                w = w * w;
+            } else if ((gmax - 2.f * gamma) <= beta && beta < (kPI * (2.f * num_rounds + 1.f) - 2.f * gamma)) {
+                w = 1.f;
+            } else if ((kPI * (2.f * num_rounds + 1.f) - 2.f * gamma) <= beta &&
+                       beta <= (kPI * (2.f * num_rounds + 1.f) + gmax)) {
+                w = sinf(kPI / 2.f * (kPI + gmax - (beta - kPI * 2.f * num_rounds)) / (gmax + 2.f * gamma));
+                w = w * w;
+            }
+        } else {
+            w = 1.f;
+        }
+        p *= w;
+    }
+    sgm[idx] = p;
+}
+
+// run_mgfbp.py:1019-1031 (vertical Gaussian, +-10 rows, ascending n).
+__global__ void vertical_kernel(const float *__restrict__ in, float *__restrict__ out, int rows,
+                                int views, int nu, const float *__restrict__ kern, float det_h) {
+    int64_t plane = (int64_t)views * nu;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= plane * rows) return;
+    int i = (int)(idx / plane);
+    int64_t jk = idx % plane;
+    int n0 = i - 10 < 0 ? 0 : i - 10;
+    int n1 = i + 10 > rows - 1 ? rows - 1 : i + 10;
+    float acc = 0.f;
+    for (int n = n0; n <= n1; ++n) acc += in[n * plane + jk] * kern[i + (rows - 1) - n];
+    out[idx] = acc * det_h;
+}
+
+template <int DT> __device__ __forceinline__ uint16_t to_half_bits(float f) {
+    return DT == MCT_F16 ? mct::f32_to_f16_rn(f) : mct::f32_to_bf16_rn(f);
+}
+
+// QUAD: out[j][r][c] = {in[r][j][c], in[r][j][c+1], in[r+1][j][c], in[r+1][j][c+1]}
+template <int DT>
+__global__ void pack_quad_kernel(const float *__restrict__ in, int rows, int views, int nu,
+                                 void *__restrict__ out) {
+    int nc = nu - 1, nr = rows - 1;
+    int64_t n = (int64_t)views * nr * nc;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    int c = (int)(idx % nc);
+    int r = (int)((idx / nc) % nr);
+    int j = (int)(idx / ((int64_t)nc * nr));
+    const float *p0 = in + ((int64_t)r * views + j) * nu + c;
+    const float *p1 = p0 + (int64_t)views * nu;
+    float a = p0[0], b = p0[1], cc = p1[0], d = p1[1];
+    if (DT == MCT_F32) {
+        reinterpret_cast<float4 *>(out)[idx] = make_float4(a, b, cc, d);
+    } else {
+        ushort4 h;
+        h.x = to_half_bits<DT>(a);
+        h.y = to_half_bits<DT>(b);
+        h.z = to_half_bits<DT>(cc);
+        h.w = to_half_bits<DT>(d);
+        reinterpret_cast<ushort4 *>(out)[idx] = h;
+    }
+}
+
+// PAIR: out[r][j][c] = {in[r][j][c], in[r][j][c+1]}
+template <int DT>
+__global__ void pack_pair_kernel(const float *__restrict__ in, int rows, int views, int nu,
+                                 void *__restrict__ out) {
+    int nc = nu - 1;
+    int64_t n = (int64_t)rows * views * nc;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    int64_t rowbase = idx / nc;  // r*views + j
+    int c = (int)(idx % nc);
+    const float *p = in + rowbase * nu + c;
+    float a = p[0], b = p[1];
+    if (DT == MCT_F32) {
+        reinterpret_cast<float2 *>(out)[idx] = make_float2(a, b);
+    } else {
+        ushort2 h;
+        h.x = to_half_bits<DT>(a);
+        h.y = to_half_bits<DT>(b);
+        reinterpret_cast<ushort2 *>(out)[idx] = h;
+    }
+}
+
+template <int DT>
+__global__ void pack_plain_kernel(const float *__restrict__ in, int64_t n, void *__restrict__ out) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    if (DT == MCT_F32)
+        reinterpret_cast<float *>(out)[idx] = in[idx];
+    else
+        reinterpret_cast<uint16_t *>(out)[idx] = to_half_bits<DT>(in[idx]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mct_weight_sinogram(mct_plan *plan, float *sgm, const float *bh_coef, int32_t bh_order) {
+    MCT_CHECK(plan && sgm, "NULL argument");
+    MCT_CHECK(bh_order >= 0 && bh_order <= kMaxBH, "bh_order must be in [0, %d]", kMaxBH);
+    MCT_CHECK(bh_order == 0 || bh_coef, "bh_coef is NULL");
+    const mct_geometry &g = plan->g;
+    mct_ctx *ctx = plan->ctx;
+    BHCoef c = {};
+    for (int t = 0; t < bh_order; ++t) c.c[t] = bh_coef[t];
+    int64_t n = (int64_t)g.n_v * g.n_views * g.n_u;
+    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                       sgm, g.n_v, g.n_views, g.n_u, plan->u, plan->v, plan->angles, g.sdd,
+                       g.total_scan_angle, g.short_scan, g.curved, c, bh_order);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+int mct_filter_vertical(mct_ctx *ctx, const float *in, float *out, int32_t rows, int32_t views,
+                        int32_t n_u, const float *kern, float det_h) {
+    MCT_CHECK(ctx && in && out && kern, "NULL argument");
+    MCT_CHECK(in != out, "vertical filter cannot run in place");
+    MCT_CHECK(rows > 0 && views > 0 && n_u > 0, "empty sinogram");
+    int rc = mct::ensure_buffer(ctx, &ctx->vkern_dev, &ctx->vkern_cap, 2 * (int64_t)rows - 1);
+    if (rc) return rc;
+    MCT_HIP(hipMemcpyAsync(ctx->vkern_dev, kern, sizeof(float) * (2 * (size_t)rows - 1),
+                           hipMemcpyHostToDevice, ctx->stream));
+    int64_t n = (int64_t)rows * views * n_u;
+    hipLaunchKernelGGL(vertical_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                       in, out, rows, views, n_u, ctx->vkern_dev, det_h);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+int64_t mct_packed_bytes(int32_t rows, int32_t views, int32_t n_u, int32_t dtype, int32_t layout) {
+    if (rows <= 0 || views <= 0 || n_u <= 0) return -1;
+    int64_t es = dtype == MCT_F32 ? 4 : 2;
+    switch (layout) {
+        case MCT_PLAIN: return (int64_t)rows * views * n_u * es;
+        case MCT_QUAD:
+            if (rows < 2 || n_u < 2) return -1;
+            return (int64_t)(rows - 1) * views * (n_u - 1) * 4 * es;
+        case MCT_PAIR:
+            if (n_u < 2) return -1;
+            return (int64_t)rows * views * (n_u - 1) * 2 * es;
+        default: return -1;
+    }
+}
+
+int mct_pack_sinogram(mct_ctx *ctx, const float *in, int32_t rows, int32_t views, int32_t n_u,
+                      void *out, int32_t dtype, int32_t layout) {
+    MCT_CHECK(ctx && in && out, "NULL argument");
+    MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
+    int64_t bytes = mct_packed_bytes(rows, views, n_u, dtype, layout);
+    MCT_CHECK(bytes > 0, "bad packed shape (rows %d, views %d, n_u %d, layout %d)", rows, views,
+              n_u, layout);
+    int64_t n;
+#define MCT_PACK_LAUNCH(K)                                                                  \
+    do {                                                                                    \
+        if (dtype == MCT_F32)                                                               \
+            hipLaunchKernelGGL(K<MCT_F32>, dim3(mct::blocks_for(n, 256)), dim3(256), 0,     \
+                               ctx->stream, ARGS);                                          \
+        else if (dtype == MCT_F16)                                                          \
+            hipLaunchKernelGGL(K<MCT_F16>, dim3(mct::blocks_for(n, 256)), dim3(256), 0,     \
+                               ctx->stream, ARGS);                                          \
+        else                                                                                \
+            hipLaunchKernelGGL(K<MCT_BF16>, dim3(mct::blocks_for(n, 256)), dim3(256), 0,    \
+                               ctx->stream, ARGS);                                          \
+    } while (0)
+    if (layout == MCT_QUAD) {
+        n = (int64_t)views * (rows - 1) * (n_u - 1);
+#define ARGS in, rows, views, n_u, out
+        MCT_PACK_LAUNCH(pack_quad_kernel);
+    } else if (layout == MCT_PAIR) {
+        n = (int64_t)rows * views * (n_u - 1);
+        MCT_PACK_LAUNCH(pack_pair_kernel);
+#undef ARGS
+    } else {
+        n = (int64_t)rows * views * n_u;
+#define ARGS in, n, out
+        MCT_PACK_LAUNCH(pack_plain_kernel);
+#undef ARGS
+    }
+#undef MCT_PACK_LAUNCH
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""Device plumbing: the C-ABI context and geometry plans over torch tensors.
+
+torch (ROCm) only allocates device memory and supplies the stream; every
+computation is a HIP kernel of libmct.so.  All calls are queued on torch's
+current stream, so they order with torch's own work on that stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import call
+from .geometry import GeometrySpec
+
+DTYPES = {"f32": _abi.MCT_F32, "f16": _abi.MCT_F16, "bf16": _abi.MCT_BF16}
+LAYOUTS = {"plain": _abi.MCT_PLAIN, "quad": _abi.MCT_QUAD, "pair": _abi.MCT_PAIR}
+FILTER_METHODS = {"f32": _abi.MCT_FILTER_F32_DIRECT, "bf16": _abi.MCT_FILTER_BF16_MFMA,
+                  "bf16x3": _abi.MCT_FILTER_BF16X3_MFMA}
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def _f32(t: torch.Tensor) -> int:
+    if t.dtype != torch.float32:
+        raise ValueError(f"expected float32, got {t.dtype}")
+    return _ptr(t)
+
+
+def _host_f32(a) -> tuple[np.ndarray, C.POINTER(C.c_float)]:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Context:
+    """An mct_ctx bound to one device, queued on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the reconstruction kernels need a ROCm device")
+        self.device = int(device)
+        h = C.c_void_p()
+        call("mct_create", self.device, C.byref(h))
+        self.h = h
+        self.bind_stream()
+
+    def bind_stream(self, stream: torch.cuda.Stream | None = None) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        call("mct_set_stream", self.h, C.c_void_p(s.cuda_stream))
+
+    def synchronize(self) -> None:
+        call("mct_synchronize", self.h)
+
+    def set_bp_tile(self, zc: int) -> None:
+        call("mct_set_bp_tile", self.h, int(zc))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _abi.lib().mct_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- context-level operations -------------------------------------------------
+    def filter_rows(self, inp: torch.Tensor, out: torch.Tensor, taps: np.ndarray, scale: float,
+                    method: str = "f32") -> torch.Tensor:
+        n = int(inp.shape[-1])
+        if taps.shape[0] != 2 * n - 1:
+            raise ValueError(f"expected {2 * n - 1} taps, got {taps.shape[0]}")
+        if out.shape != inp.shape:
+            raise ValueError("out must have the shape of inp")
+        keep, tp = _host_f32(taps)
+        call("mct_filter_rows", self.h, _f32(inp), _f32(out), inp.numel() // n, n, tp,
+             float(scale), FILTER_METHODS[method])
+        return out
+
+    def filter_vertical(self, inp: torch.Tensor, out: torch.Tensor, kern: np.ndarray,
+                        det_h: float) -> torch.Tensor:
+        rows, views, nu = inp.shape
+        keep, kp = _host_f32(kern)
+        call("mct_filter_vertical", self.h, _f32(inp), _f32(out), rows, views, nu, kp, float(det_h))
+        return out
+
+    def pack(self, inp: torch.Tensor, dtype: str, layout: str) -> torch.Tensor:
+        rows, views, nu = inp.shape
+        nbytes = _abi.lib().mct_packed_bytes(rows, views, nu, DTYPES[dtype], LAYOUTS[layout])
+        if nbytes <= 0:
+            raise ValueError(f"cannot pack shape {tuple(inp.shape)} as {layout}")
+        out = torch.empty(int(nbytes), dtype=torch.uint8, device=inp.device)
+        call("mct_pack_sinogram", self.h, _f32(inp), rows, views, nu, _ptr(out), DTYPES[dtype],
+             LAYOUTS[layout])
+        return out
+
+    def add_noise(self, sgm: torch.Tensor, photons: float, seed: int) -> torch.Tensor:
+        call("mct_add_noise", self.h, _f32(sgm), sgm.numel(), float(photons), int(seed) & (2**64 - 1))
+        return sgm
+
+    def dot(self, a: torch.Tensor, b: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        call("mct_dot_f64", self.h, _f32(a), _f32(b), a.numel(), _ptr(out))
+        return out
+
+    def sq_diff(self, a, b, mb, out):
+        call("mct_sq_diff_f64", self.h, _f32(a), _f32(b), _ptr(mb), a.numel(), _ptr(out))
+        return out
+
+    def tv_weight(self, x, prior, beta: float, out):
+        nz, ny, nx = x.shape
+        call("mct_tv_weight", self.h, _f32(x), _ptr(prior), nz, ny, nx, float(beta), _f32(out))
+        return out
+
+    def tv_value(self, x, prior, beta: float, out):
+        nz, ny, nx = x.shape
+        call("mct_tv_value", self.h, _f32(x), _ptr(prior), nz, ny, nx, float(beta), _ptr(out))
+        return out
+
+    def dtwd(self, x, wr, scale: float, base, out):
+        nz, ny, nx = x.shape
+        call("mct_dtwd", self.h, _f32(x), _f32(wr), nz, ny, nx, float(scale), _ptr(base), _f32(out))
+        return out
+
+    def axpby(self, a: float, x, b: float, y, out):
+        call("mct_axpby", self.h, float(a), _f32(x), float(b), _f32(y), _f32(out), x.numel())
+        return out
+
+    def cg_update(self, x, r, d, q, s):
+        call("mct_cg_update", self.h, _f32(x), _f32(r), _f32(d), _f32(q), _ptr(s), x.numel())
+
+    def cg_direction(self, d, r, s):
+        call("mct_cg_direction", self.h, _f32(d), _f32(r), _ptr(s), d.numel())
+
+
+class Plan:
+    """An mct_plan: a geometry uploaded once, with its per-view tables."""
+
+    def __init__(self, ctx: Context, spec: GeometrySpec):
+        self.ctx = ctx
+        self.spec = spec
+        st = spec.to_struct()
+        h = C.c_void_p()
+        call("mct_plan_create", ctx.h, C.byref(st), C.byref(h))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _abi.lib().mct_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def weight(self, sgm: torch.Tensor, bh_coef=None) -> torch.Tensor:
+        if bh_coef is None or len(bh_coef) == 0:
+            call("mct_weight_sinogram", self.h, _f32(sgm), None, 0)
+        else:
+            keep, cp = _host_f32(bh_coef)
+            call("mct_weight_sinogram", self.h, _f32(sgm), cp, len(keep))
+        return sgm
+
+    def backproject_fbp(self, filt: torch.Tensor, dtype: str, layout: str,
+                        vol: torch.Tensor) -> torch.Tensor:
+        call("mct_backproject_fbp", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
+        return vol
+
+    def forward_project(self, vol: torch.Tensor, out: torch.Tensor, ray_mask=None,
+                        oversample: int = 1) -> torch.Tensor:
+        call("mct_forward_project", self.h, _f32(vol), _f32(out), _ptr(ray_mask), int(oversample))
+        return out
+
+    def ray_mask(self, out: torch.Tensor) -> torch.Tensor:
+        call("mct_ray_mask", self.h, _f32(out))
+        return out
+
+    def backproject_ir(self, sgm: torch.Tensor, ray_mask, vol: torch.Tensor) -> torch.Tensor:
+        call("mct_backproject_ir", self.h, _f32(sgm), _ptr(ray_mask), _f32(vol))
+        return vol

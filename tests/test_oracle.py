@@ -1,0 +1,314 @@
+"""Pins the CPU oracle (oracle/recon.py, oracle/bp_oracle.c) against what the
+mathematics fixes: closed forms, library routines, published known-answer
+vectors and brute force.  The reference ships no tests or data, so these are
+the pins (each chosen so that a dropped term, a sign or index error or a
+transposed operand fails).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import cbp, recon
+from reconstruction_program_taichi_amd import geometry as geo
+from tests._cases import cone_case, fan_case, oracle_dict, rel_l2
+
+
+# ------------------------------------------------------------------ taps
+def _response(h, w, f):
+    n = (np.arange(h.shape[0]) - (h.shape[0] - 1) // 2) * w
+    return np.real(np.sum(h[None, :] * np.exp(-2j * np.pi * f[:, None] * n[None, :]), axis=1)) * w
+
+
+def test_ramlak_taps_frequency_response_is_ramp():
+    n, w = 512, 0.4
+    h = recon.ramlak_taps(n, w, np.float64)
+    f = np.linspace(0.05, 0.95, 19) / (2 * w)
+    r = _response(h, w, f)
+    assert np.allclose(r, np.abs(f), rtol=2e-3, atol=0)
+    assert abs(_response(h, w, np.array([0.0]))[0]) < 1e-3 / w  # no DC response
+
+
+@pytest.mark.parametrize("t", [1.0, 0.54, 0.0])
+def test_hamming_taps_frequency_response(t):
+    """GenerateHammingKernel blends the ramp with the ramp times a cosine
+    window: response |f| (t + (1-t) cos(pi f w)) on |f| < 1/(2w)."""
+    n, w = 512, 0.4
+    h = geo.hamming_taps(n, w, t, 1000.0).astype(np.float64)
+    f = np.linspace(0.05, 0.9, 18) / (2 * w)
+    r = _response(h, w, f)
+    expect = np.abs(f) * (t + (1 - t) * np.cos(np.pi * f * w))
+    assert np.allclose(r, expect, rtol=5e-3, atol=2e-3 / (2 * w))
+
+
+def test_hamming_t1_equals_ramlak_oracle():
+    h32 = geo.hamming_taps(300, 0.4, 1.0, 1000.0)
+    assert np.array_equal(h32, recon.ramlak_taps(300, 0.4, np.float32))
+
+
+def test_gaussian_apodized_ramp_is_convolution():
+    n, w, d = 64, 0.5, 1.5
+    full = geo.gaussian_apodized_ramp_taps(n, w, d, 1000.0)
+    g = geo.gaussian_taps(n, w, d).astype(np.float64)
+    r = geo.hamming_taps(n, w, 1.0, 1000.0).astype(np.float64)
+    brute = np.array([sum(g[j] * r[i - (j - (n - 1))] for j in range(2 * n - 1)
+                          if 0 <= i - (j - (n - 1)) <= 2 * n - 2) for i in range(2 * n - 1)]) * w
+    assert np.allclose(full, brute, rtol=1e-5, atol=1e-7)
+
+
+# ---------------------------------------------------------------- filter
+def test_filter_rows_equals_dense_toeplitz_and_convolve():
+    rng = np.random.default_rng(0)
+    n = 57
+    x = rng.standard_normal((5, n))
+    taps = rng.standard_normal(2 * n - 1)
+    T = np.array([[taps[k + n - 1 - m] for k in range(n)] for m in range(n)])
+    ref = x @ T * 0.7
+    assert np.allclose(recon.filter_rows(x, taps, 0.7, np.float64), ref, rtol=1e-12, atol=1e-12)
+    got32 = recon.filter_rows(x.astype(np.float32), taps.astype(np.float32), 0.7, np.float32)
+    assert rel_l2(got32, ref) < 1e-6
+
+
+# ------------------------------------------------------------- weighting
+def _parker(beta, gamma, scan):
+    """Parker weight through the oracle: unit sinogram, cosine weight divided out."""
+    sdd = 1000.0
+    u = np.array([-np.tan(gamma) * sdd], np.float64)
+    ang = np.array([0.0, beta], np.float64)
+    s = np.ones((1, 2, 1))
+    w = recon.weight_sinogram(s, u, np.zeros(1), ang, sdd, scan, True, dtype=np.float64)
+    cosw = sdd * sdd / np.sqrt(sdd * sdd + u[0] ** 2)
+    return w[0, 1, 0] / cosw
+
+
+def test_parker_weights_are_complementary():
+    rng = np.random.default_rng(1)
+    gmax = 0.25
+    scan = np.pi + gmax  # recon.PI differs from pi in the 11th digit; use the reference's PI
+    scan = recon.PI + gmax
+    for _ in range(50):
+        g = rng.uniform(-gmax / 2 * 0.95, gmax / 2 * 0.95)
+        b = rng.uniform(0, recon.PI - 2 * g)
+        w1 = _parker(b, g, scan)
+        w2 = _parker(b + recon.PI + 2 * g, -g, scan)
+        assert abs(w1 + w2 - 1.0) < 1e-6
+
+
+def test_full_scan_weight_is_cosine_only():
+    spec = fan_case()
+    s = np.ones((spec.n_v, spec.n_views, spec.n_u), np.float32)
+    w = recon.weight_sinogram(s, spec.u, spec.v, spec.angles, spec.sdd, spec.total_scan_angle, False,
+                              dtype=np.float64)
+    u = spec.u.astype(np.float64)
+    assert np.allclose(w[0, 3], spec.sdd ** 2 / np.sqrt(spec.sdd ** 2 + u ** 2))
+
+
+# -------------------------------------------------------- back-projection
+def _bp_brute(filt, g):
+    """Scalar loops of run_mgfbp.py:1086-1182 on a tiny grid (FP32 scalars)."""
+    f32 = np.float32
+    nx, nz = g["img_dim"], g["img_dim_z"]
+    nrows, nviews, nu = filt.shape
+    c, s, d = recon.view_table(g["angles"], 0.0, np.float32)
+    div = recon.div_factor(g["total_scan_angle"], False, np.float32)
+    u, v = g["u"], g["v"]
+    out = np.zeros((nz, nx, nx), np.float32)
+    for iz in range(nz):
+        for iy in range(nx):
+            for ix in range(nx):
+                x = f32(g["pix"]) * (f32(ix) - f32((nx - 1) / 2.0))
+                y = (-f32(g["pix"])) * (f32(iy) - f32((nx - 1) / 2.0))
+                z = (f32(iz) - f32((nz - 1) / 2.0)) * f32(g["vox_h"]) + f32(g.get("cz", 0.0))
+                acc = f32(0)
+                for j in range(nviews):
+                    L = f32(g["sid"]) - x * c[j] - y * s[j]
+                    mag = f32(g["sdd"]) / L
+                    ui = (mag * ((-x) * s[j] + y * c[j]) - u[0]) / (u[1] - u[0])
+                    if ui < 0 or ui + f32(1) > f32(nu - 1):
+                        acc = None
+                        break
+                    uf = int(np.floor(ui))
+                    ru = ui - f32(uf)
+                    vi = (mag * z - v[0]) / f32(g["det_h"]) * np.abs(v[1] - v[0]) / (v[1] - v[0])
+                    vf = int(np.floor(vi))
+                    if vf < 0 or vf + 1 > nrows - 1:
+                        acc = None
+                        break
+                    rv = vi - f32(vf)
+                    p0 = filt[vf, j, uf] * (f32(1) - ru) + filt[vf, j, uf + 1] * ru
+                    p1 = filt[vf + 1, j, uf] * (f32(1) - ru) + filt[vf + 1, j, uf + 1] * ru
+                    acc = acc + (f32(g["sid"]) * (f32(1) / (L * L))) * ((f32(1) - rv) * p0 + rv * p1) * d[j] * div
+                out[iz, iy, ix] = 0 if acc is None else acc
+    return out
+
+
+def test_bp_oracle_matches_brute_force_tiny():
+    spec = cone_case(nx=6, nz=5, nv=8, nu=20, nrows=10, pix=2.0, vh=2.0, w=1.6, h=1.6)
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((spec.n_v, spec.n_views, spec.n_u)).astype(np.float32)
+    a = recon.backproject_fbp(f, g, np.float32)
+    b = _bp_brute(f, g)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["cone", "fan"])
+def test_c_oracle_matches_numpy_oracle(kind):
+    spec = cone_case() if kind == "cone" else fan_case()
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((spec.n_v, spec.n_views, spec.n_u)).astype(np.float32)
+    assert np.array_equal(recon.backproject_fbp(f, g, np.float32), cbp.backproject_fbp(f, g))
+
+
+def _disc_fbp(nviews=360, nu=256, nx=64):
+    """FBP (FP64 oracle) of the analytic fan-beam sinogram of a uniform disc."""
+    R, mu = 10.0, 0.02
+    pix, w, sid, sdd = 0.5, 0.25, 200.0, 400.0
+    spec = fan_case(nx=nx, nv=nviews, nu=nu, pix=pix, w=w, sid=sid, sdd=sdd, rows=1)
+    u = spec.u.astype(np.float64)
+    gam = np.arctan2(u, sdd)
+    dd = sid * np.abs(np.sin(gam))  # distance of each ray from the isocentre
+    chord = 2 * np.sqrt(np.clip(R * R - dd * dd, 0, None)) * mu
+    sgm = np.broadcast_to(chord[None, None, :], (1, nviews, nu)).astype(np.float64)
+    ws = recon.weight_sinogram(sgm, spec.u, spec.v, spec.angles, sdd, spec.total_scan_angle, False,
+                               dtype=np.float64)
+    fl = recon.filter_rows(ws, recon.ramlak_taps(nu, w), w, np.float64)
+    g = oracle_dict(spec)
+    return recon.backproject_fbp(fl, g, np.float64), mu, R, pix, nx
+
+
+def test_fbp_of_disc_reproduces_disc():
+    img, mu, R, pix, nx = _disc_fbp()
+    c = (nx - 1) / 2.0
+    yy, xx = np.mgrid[0:nx, 0:nx]
+    r = np.hypot((xx - c) * pix, (yy - c) * pix)
+    inner = img[0][r < 0.7 * R]
+    outer = img[0][(r > 1.3 * R) & (r < 14.0)]
+    assert abs(inner.mean() / mu - 1) < 0.02
+    assert inner.std() / mu < 0.03
+    assert np.abs(outer).mean() / mu < 0.03
+
+
+# ---------------------------------------------------- forward projection
+def test_fp_of_uniform_disc_equals_chord_length():
+    """FP (FP64 oracle) of a slice holding a centred uniform disc: chord
+    length times mu, within the sampling/interpolation error."""
+    nx, pix, R, mu = 96, 0.5, 18.0, 0.02
+    c = (nx - 1) / 2.0
+    yy, xx = np.mgrid[0:nx, 0:nx]
+    ss = 4
+    img = np.zeros((nx, nx))
+    for a in range(ss):
+        for b in range(ss):
+            img += np.hypot((xx - c + (a + 0.5) / ss - 0.5) * pix, (yy - c + (b + 0.5) / ss - 0.5) * pix) <= R
+    img = img / ss ** 2 * mu
+    spec = fan_case(nx=nx, nv=12, nu=160, pix=pix, w=0.35, sid=150.0, sdd=300.0, rows=1)
+    g = oracle_dict(spec)
+    sgm = recon.forward_project(img[None], g, dtype=np.float64)
+    u = spec.u.astype(np.float64)
+    dd = 150.0 * np.abs(np.sin(np.arctan2(u, 300.0)))
+    chord = 2 * np.sqrt(np.clip(R * R - dd * dd, 0, None)) * mu
+    inside = dd < R - 1.0
+    err = np.abs(sgm[0][:, inside] - chord[None, inside]) / (chord[None, inside])
+    assert err.max() < 0.02
+
+
+def test_fp_oracle_fp32_close_to_fp64():
+    spec = cone_case(nx=24, nz=12, nv=10, nu=40, nrows=16)
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(4)
+    vol = rng.uniform(0, 1, (12, 24, 24))
+    a = recon.forward_project(vol.astype(np.float32), g, dtype=np.float32)
+    b = recon.forward_project(vol, g, dtype=np.float64)
+    assert rel_l2(a, b) < 1e-5
+
+
+def test_fp_is_linear():
+    spec = fan_case(nx=32, nv=8, nu=50)
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(5)
+    a, b = rng.uniform(0, 1, (2, 32, 32)), rng.uniform(0, 1, (2, 32, 32))
+    fa = recon.forward_project(a, g, dtype=np.float64)
+    fb = recon.forward_project(b, g, dtype=np.float64)
+    fab = recon.forward_project(2 * a - 3 * b, g, dtype=np.float64)
+    assert np.allclose(fab, 2 * fa - 3 * fb, atol=1e-10)
+
+
+# ------------------------------------------------------------------ RNG
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors (Salmon et al. 2011, Random123)."""
+    cases = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in cases:
+        got = recon.philox4x32_10(np.array([ctr], np.uint32), np.array([key], np.uint32))[0]
+        assert [int(x) for x in got] == want
+
+
+def test_normals_statistics():
+    g = recon.normals(200000, seed=7)
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1) < 0.01
+
+
+# ---------------------------------------------------------- CG / TV
+def test_cg_recursion_solves_spd_system():
+    rng = np.random.default_rng(6)
+    M = rng.standard_normal((30, 30))
+    A = M @ M.T + 30 * np.eye(30)
+    b = rng.standard_normal(30)
+    x = recon.cg_solve(lambda v: A @ v, b, np.zeros(30), 40)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-10)
+
+
+def _diff_op(n):
+    """Forward difference with a zero last row (the reference's D)."""
+    D = sp.lil_matrix((n, n))
+    for i in range(n - 1):
+        D[i, i], D[i, i + 1] = -1, 1
+    return D.tocsr()
+
+
+@pytest.mark.parametrize("nz", [1, 2, 5])
+def test_dtwd_matches_sparse_operator_interior(nz):
+    ny, nx = 7, 6
+    rng = np.random.default_rng(nz)
+    x = rng.standard_normal((nz, ny, nx))
+    wr = rng.uniform(0.5, 2, (nz, ny, nx))
+    I = lambda n: sp.identity(n, format="csr")
+    Dy = sp.kron(sp.kron(I(nz), _diff_op(ny)), I(nx))
+    Dx = sp.kron(sp.kron(I(nz), I(ny)), _diff_op(nx))
+    W = sp.diags(wr.ravel())
+    ops = [Dy, Dx]
+    if nz > 2:
+        ops.append(sp.kron(sp.kron(_diff_op(nz), I(ny)), I(nx)))
+    full = sum((D.T @ (W @ (D @ x.ravel()))) for D in ops).reshape(x.shape)
+    got = recon.dtwd(x, wr)
+    # The reference's operator equals D^T W D except that each axis' first and
+    # last index receive no contribution from that axis.
+    sl = (slice(1, -1) if nz > 2 else slice(None), slice(1, -1), slice(1, -1))
+    assert np.allclose(got[sl], full[sl], atol=1e-12)
+    if nz <= 2:  # row y = 0 only sees the x-axis term
+        xonly = (Dx.T @ (W @ (Dx @ x.ravel()))).reshape(x.shape)
+        assert np.allclose(got[:, 0, 1:-1], xonly[:, 0, 1:-1], atol=1e-12)
+
+
+def test_tv_weight_and_value():
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((3, 4, 5))
+    beta = 1e-3
+    ux = np.zeros_like(x)
+    ux[:, :-1] = np.diff(x, axis=1)
+    uy = np.zeros_like(x)
+    uy[:, :, :-1] = np.diff(x, axis=2)
+    uz = np.zeros_like(x)
+    uz[:-1] = np.diff(x, axis=0)
+    mag = np.sqrt(ux ** 2 + uy ** 2 + uz ** 2 + beta)
+    assert np.allclose(recon.tv_weight(x, beta), 1 / mag)
+    assert np.isclose(recon.tv_value(x, beta), mag.sum())
+    # two slices: no z coupling (reference: only when Nz > 2)
+    x2 = x[:2]
+    mag2 = np.sqrt(ux[:2] ** 2 + uy[:2] ** 2 + beta)
+    assert np.allclose(recon.tv_weight(x2, beta), 1 / mag2)
