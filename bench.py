@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--filter", default="f32", choices=["f32", "bf16", "bf16x3"])
     ap.add_argument("--cpu-slices", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bp-tile", type=int, default=0, help="slices per thread (0: library default)")
+    ap.add_argument("--bp-batch", type=int, default=0, help="loads per batch (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -94,6 +96,8 @@ def main() -> int:
     work = torch.empty_like(raw)
     vol = torch.empty((nz, nx, nx), dtype=torch.float32, device=dev)
     plan = fbp.plan()
+    if args.bp_tile or args.bp_batch:
+        fbp.ctx().set_bp_tile(args.bp_tile, args.bp_batch)
     stream = torch.cuda.current_stream(dev)
 
     bp_times = []

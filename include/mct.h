@@ -164,9 +164,10 @@ int64_t mct_packed_bytes(int32_t rows, int32_t views, int32_t n_u, int32_t dtype
 int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
                         float *vol);
 
-/* Slices per thread of the axial back-projector (tile autotuner hook):
- * 4, 8, 16 or 32; 0 restores the default. */
-int mct_set_bp_tile(mct_ctx *ctx, int32_t zc);
+/* Tile shape of the axial back-projector (autotuner hook): zc slices per
+ * thread (4, 8, 16 or 32) and, for cone beam, batch taps loads issued
+ * together (4, 8 or 16, at most zc); 0 restores a default. */
+int mct_set_bp_tile(mct_ctx *ctx, int32_t zc, int32_t batch);
 
 /* ---- forward projection -------------------------------------------------- */
 

@@ -72,7 +72,7 @@ SIGNATURES = {
     "mct_pack_sinogram": (C.c_int, [_P, _P, _I32, _I32, _I32, _P, _I32, _I32]),
     "mct_packed_bytes": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "mct_backproject_fbp": (C.c_int, [_P, _P, _I32, _I32, _P]),
-    "mct_set_bp_tile": (C.c_int, [_P, _I32]),
+    "mct_set_bp_tile": (C.c_int, [_P, _I32, _I32]),
     "mct_forward_project": (C.c_int, [_P, _P, _P, _P, _I32]),
     "mct_ray_mask": (C.c_int, [_P, _P]),
     "mct_add_noise": (C.c_int, [_P, _P, _I64, _F32, _U64]),

@@ -5,24 +5,31 @@
 // (run_mgfbp_iterative_recon_IRN_CGLS.py:372-481).
 //
 // Axial fast path (no P-matrix): one thread owns an (x, y) column and ZC
-// consecutive slices of it.  Everything that depends only on (column, view)
-// - the source distance L, the magnification, the detector u index and the
-// view weight - is computed once per view and reused for the ZC slices, so a
-// voxel-view update costs one v index, one vector load of the taps and three
-// FMAs.  The filtered sinogram arrives pre-packed so that the taps of one
-// update are a single 16-byte (FP32) or 8-byte (FP16/BF16) load: QUAD layout
-// [view][row-1][col-1] for cone beam, PAIR layout [row][view][col-1] for fan
-// beam.  Workgroups are remapped so that each XCD owns a contiguous range of
-// slice chunks: its L2 then only holds the detector rows that range projects
-// onto.
+// consecutive slices of it; a wavefront covers 8 x 8 columns.  Everything
+// that depends only on (column, view) - the source distance L, the
+// magnification, the detector u position, the view weight and the range of
+// slices whose projection stays on the detector - is computed once per view,
+// so the slice loop is branch-free: per voxel-view update one FMA for the
+// detector row coordinate, a clamp, a convert, a fract, one address
+// multiply-add, one vector load of the packed taps and four FMAs.  All ZC
+// loads of a view are independent and in flight together.
+//
+// The filtered sinogram arrives pre-packed as bilinear coefficients, so the
+// taps of one update are a single 16-byte (FP32) or 8-byte (FP16/BF16) load:
+// QUAD layout [view][row-1][col-1] x {a, b-a, c-a, d-c-b+a} for cone beam,
+// PAIR layout [row][view][col-1] x {a, b-a} for fan beam.  Workgroups are
+// remapped so that each XCD owns a contiguous range of slice chunks: its L2
+// then only holds the detector rows that range projects onto.
 //
 // Mask semantics (run_mgfbp.py:1142-1144, 1170-1173): a voxel whose
-// projection leaves the detector in any view is 0.  The decisions follow the
-// reference's FP32 expression order (the library is built with
-// -ffp-contract=off and IEEE division): the u test once per column and view;
-// the v test through a reciprocal multiply whose sign always agrees with the
-// reference, redone with the reference's divisions within 1e-3 of the upper
-// edge.  The interpolation itself uses explicit FMAs.
+// projection leaves the detector in any view is 0.  The geometry plan turns
+// the reference's FP32 range tests into exact intervals on u = fl(mag * yr)
+// and d = fl(fl(mag * z) - v[0]) (both tests are monotone in those values),
+// so the kernels decide the mask exactly as the reference's FP32 arithmetic
+// does (the library is built with -ffp-contract=off; mag = sdd / L is an
+// IEEE division).  Because d is monotone in the slice index, the slices that
+// pass the v test in a view form an interval; it is located with the fast
+// row coordinate and its two ends are confirmed with the exact test.
 #include "mct_internal.h"
 
 namespace {
@@ -31,7 +38,8 @@ struct BpArgs {
     int nx, nz, nu, nv, nviews;
     float pix, vh, cx, cy, cz, cos_rot, sin_rot;
     float sid, sdd, div_factor, det_h;
-    float u0, du, v0, dv, adv, vinv;  // vinv = adv / (det_h * dv)
+    float u0, du, inv_du, v0, dv, adv, vinv, vlim;
+    float ua, ub, da, db;  // exact range-test intervals (see the file comment)
     int curved, cone, view_mode, apply_pm;
     const float *pm, *center_z;
     const float4 *vt;  // {cos, sin, dbeta, 0} of (angle - img_rot) per view
@@ -44,8 +52,17 @@ struct BpArgs {
 
 template <int DT> struct Taps;
 template <> struct Taps<MCT_F32> {
+    static constexpr int kQuadBytes = 16, kPairBytes = 8;
+    using Raw = float4;
+    __device__ __forceinline__ static Raw load_raw(const char *p) {
+        return *reinterpret_cast<const float4 *>(p);
+    }
+    __device__ __forceinline__ static float4 cvt(float4 q) { return q; }
     __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
         return reinterpret_cast<const float4 *>(base)[i];
+    }
+    __device__ __forceinline__ static float4 quad_at(const char *p) {
+        return *reinterpret_cast<const float4 *>(p);
     }
     __device__ __forceinline__ static float2 pair(const void *base, size_t i) {
         return reinterpret_cast<const float2 *>(base)[i];
@@ -55,10 +72,20 @@ template <> struct Taps<MCT_F32> {
     }
 };
 template <> struct Taps<MCT_F16> {
-    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
-        ushort4 h = reinterpret_cast<const ushort4 *>(base)[i];
+    static constexpr int kQuadBytes = 8, kPairBytes = 4;
+    using Raw = ushort4;
+    __device__ __forceinline__ static Raw load_raw(const char *p) {
+        return *reinterpret_cast<const ushort4 *>(p);
+    }
+    __device__ __forceinline__ static float4 cvt(ushort4 h) {
         return make_float4(mct::f16_to_f32(h.x), mct::f16_to_f32(h.y), mct::f16_to_f32(h.z),
                            mct::f16_to_f32(h.w));
+    }
+    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
+        return cvt(reinterpret_cast<const ushort4 *>(base)[i]);
+    }
+    __device__ __forceinline__ static float4 quad_at(const char *p) {
+        return cvt(*reinterpret_cast<const ushort4 *>(p));
     }
     __device__ __forceinline__ static float2 pair(const void *base, size_t i) {
         ushort2 h = reinterpret_cast<const ushort2 *>(base)[i];
@@ -69,10 +96,20 @@ template <> struct Taps<MCT_F16> {
     }
 };
 template <> struct Taps<MCT_BF16> {
-    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
-        uint2 w = reinterpret_cast<const uint2 *>(base)[i];
+    static constexpr int kQuadBytes = 8, kPairBytes = 4;
+    using Raw = uint2;
+    __device__ __forceinline__ static Raw load_raw(const char *p) {
+        return *reinterpret_cast<const uint2 *>(p);
+    }
+    __device__ __forceinline__ static float4 cvt(uint2 w) {
         return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u),
                            __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xffff0000u));
+    }
+    __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
+        return cvt(reinterpret_cast<const uint2 *>(base)[i]);
+    }
+    __device__ __forceinline__ static float4 quad_at(const char *p) {
+        return cvt(*reinterpret_cast<const uint2 *>(p));
     }
     __device__ __forceinline__ static float2 pair(const void *base, size_t i) {
         uint32_t w = reinterpret_cast<const uint32_t *>(base)[i];
@@ -83,14 +120,21 @@ template <> struct Taps<MCT_BF16> {
     }
 };
 
-// Logical tile of this workgroup: 16x16 columns and one chunk of ZC slices.
+// Bilinear value from packed coefficients {a, b-a, c-a, d-c-b+a}.
+__device__ __forceinline__ float bilerp(float4 q, float ru, float rv) {
+    return fmaf(rv, fmaf(ru, q.w, q.z), fmaf(ru, q.y, q.x));
+}
+
+// Logical tile of this workgroup: 16 x 16 columns (four 8 x 8 wavefront
+// tiles) and one chunk of ZC slices.
 __device__ __forceinline__ void decode_tile(const BpArgs &a, int &ix, int &iy, int &zc_idx) {
     int nb = a.tiles_x * a.tiles_y * a.nzc;
     int b = mct::xcd_remap((int)blockIdx.x, nb, a.nxcd);
     int t = b % (a.tiles_x * a.tiles_y);
     zc_idx = b / (a.tiles_x * a.tiles_y);
-    ix = (t % a.tiles_x) * 16 + (int)(threadIdx.x & 15);
-    iy = (t / a.tiles_x) * 16 + (int)(threadIdx.x >> 4);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    ix = (t % a.tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    iy = (t / a.tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
 // In-plane position of voxel column (ix, iy) after ImageRotation (axial
@@ -105,102 +149,132 @@ __device__ __forceinline__ void voxel_xy(const BpArgs &a, int ix, int iy, float 
 
 struct ColumnView {
     bool ok;
-    int uf;
+    uint32_t uf;
     float ru, mag, coef;
 };
 
-// Per (column, view) geometry of run_mgfbp.py:1122-1155.
+// Per (column, view) geometry of run_mgfbp.py:1122-1155 for a flat
+// detector; the u range test is exact (ua <= u <= ub).
 __device__ __forceinline__ ColumnView column_view(const BpArgs &a, float x, float y, float4 t) {
     ColumnView cv;
     float L = a.sid - x * t.x - y * t.y;
     cv.mag = a.sdd / L;
     float yr = (-x) * t.y + y * t.x;
-    float u = a.curved ? a.sdd * atan2f(yr, L) : cv.mag * yr;
-    float uidx = (u - a.u0) / a.du;
-    cv.ok = !(uidx < 0.f || uidx + 1.f > (float)(a.nu - 1));
-    float uff = floorf(uidx);
-    int uf = (int)uff;
-    uf = uf < 0 ? 0 : (uf > a.nu - 2 ? a.nu - 2 : uf);
-    cv.uf = uf;
-    cv.ru = uidx - (float)uf;
+    float u;
     float w;
     if (a.curved) {
+        u = a.sdd * atan2f(yr, L);
         float q = x * t.y - y * t.x;
-        w = 1.0f / ((L * L) + q * q);
+        w = __builtin_amdgcn_rcpf((L * L) + q * q);
     } else {
-        w = 1.0f / (L * L);
+        u = cv.mag * yr;
+        w = __builtin_amdgcn_rcpf(L * L);
     }
-    cv.coef = (a.sid * w) * t.z * a.div_factor;
+    cv.ok = u >= a.ua && u <= a.ub;
+    float uidx = (u - a.u0) * a.inv_du;
+    int uf = (int)uidx;  // uidx >= -tiny when ok: truncation is the floor
+    uf = uf < 0 ? 0 : (uf > a.nu - 2 ? a.nu - 2 : uf);
+    cv.uf = (uint32_t)uf;
+    cv.ru = uidx - (float)uf;
+    cv.coef = w * t.w;  // t.w = sid * dbeta * div_factor
     return cv;
 }
 
-// Detector row index of run_mgfbp.py:1157-1164 and its range test
-// (run_mgfbp.py:1170-1173).  Returns false when the voxel is masked.
-__device__ __forceinline__ bool v_index(const BpArgs &a, float vproj, float vmax, int &vf,
-                                        float &rv) {
-    float d = vproj - a.v0;
-    float t = d * a.vinv;
-    if (t < 0.f) return false;  // same sign as the reference's quotient
-    if (t > vmax - 1e-3f) {
-        float te = d / a.det_h * a.adv / a.dv;
-        if (floorf(te) + 1.f > vmax) return false;
-    }
-    int f = (int)t;
-    f = f > a.nv - 2 ? a.nv - 2 : f;
-    vf = f;
-    rv = t - (float)f;
-    return true;
+// Exact v range test of slice iz: d = fl(fl(mag * z) - v0) in [da, db].
+__device__ __forceinline__ bool v_exact(const BpArgs &a, float mag, int iz, float halfz) {
+    float z = ((float)iz - halfz) * a.vh + a.cz;
+    float d = mag * z - a.v0;
+    return d >= a.da && d <= a.db;
 }
 
-// Cone beam, axial view, QUAD taps; ZC slices per thread.
-template <int ZC, int DT>
+// Slices [lo, hi] of the chunk starting at z0 (nk slices) whose projection
+// passes the v test in this view.  t(k) = t0 + k dt is the fast row
+// coordinate; the ends found from it are confirmed with the exact test.
+__device__ __forceinline__ void v_interval(const BpArgs &a, float mag, int z0, int nk, float halfz,
+                                           float t0, float dt, int &lo, int &hi) {
+    const float vmax = (float)(a.nv - 1);
+    float k_at0 = -t0 / dt, k_atm = (vmax - t0) / dt;
+    float kf_lo = dt > 0.f ? k_at0 : k_atm, kf_hi = dt > 0.f ? k_atm : k_at0;
+    // first k with t in range, last k with t in range (fast)
+    kf_lo = fminf(fmaxf(kf_lo, -2.f), (float)nk + 2.f);
+    kf_hi = fminf(fmaxf(kf_hi, -2.f), (float)nk + 2.f);
+    int l = (int)ceilf(kf_lo);
+    int h = (int)ceilf(kf_hi) - 1;
+    // confirm the ends with the exact test (the fast coordinate is off by far
+    // less than one slice, so each end moves by at most one)
+    if (l - 1 >= 0 && l - 1 < nk && v_exact(a, mag, z0 + l - 1, halfz))
+        l -= 1;
+    else if (l >= 0 && l < nk && !v_exact(a, mag, z0 + l, halfz))
+        l += 1;
+    if (h + 1 >= 0 && h + 1 < nk && v_exact(a, mag, z0 + h + 1, halfz))
+        h += 1;
+    else if (h >= 0 && h < nk && !v_exact(a, mag, z0 + h, halfz))
+        h -= 1;
+    lo = l;
+    hi = h;
+}
+
+// Cone beam, axial view, QUAD taps; ZC slices per thread, loads in batches
+// of B.
+template <int ZC, int B, int DT>
 __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
+    static_assert(ZC % B == 0, "batch must divide the slice chunk");
     int ix, iy, zci;
     decode_tile(a, ix, iy, zci);
     if (ix >= a.nx || iy >= a.nx) return;
     const int z0 = zci * ZC;
+    const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
     float x, y;
     voxel_xy(a, ix, iy, x, y);
     const float halfz = (float)(a.nz - 1) / 2.0f;
-    float zs[ZC], acc[ZC];
+    const float zfirst = ((float)z0 - halfz) * a.vh + a.cz;
+    const float zstep_t = a.vh * a.vinv;
+    float acc[ZC];
 #pragma unroll
-    for (int k = 0; k < ZC; ++k) {
-        zs[k] = ((float)(z0 + k) - halfz) * a.vh + a.cz;
-        acc[k] = 0.f;
-    }
-    const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
-    uint32_t alive = nk >= 32 ? 0xffffffffu : ((1u << nk) - 1u);
-    const float vmax = (float)(a.nv - 1);
-    const int ncol = a.nu - 1;
-    const size_t view_stride = (size_t)(a.nv - 1) * (size_t)ncol;
-    for (int j = 0; j < a.nviews && alive; ++j) {
+    for (int k = 0; k < ZC; ++k) acc[k] = 0.f;
+    int lo = 0, hi = nk - 1;
+    const uint32_t rowbytes = (uint32_t)(a.nu - 1) * Taps<DT>::kQuadBytes;
+    const size_t viewbytes = (size_t)(a.nv - 1) * rowbytes;
+    const char *sino = reinterpret_cast<const char *>(a.sino);
+    for (int j = 0; j < a.nviews; ++j) {
         const float4 t = a.vt[j];
         ColumnView cv = column_view(a, x, y, t);
         if (!cv.ok) {
-            alive = 0;
+            hi = -1;
             break;
         }
-        const size_t vbase = (size_t)j * view_stride + (size_t)cv.uf;
+        const float t0 = (cv.mag * zfirst - a.v0) * a.vinv;
+        const float dt = cv.mag * zstep_t;
+        int l, h;
+        v_interval(a, cv.mag, z0, nk, halfz, t0, dt, l, h);
+        lo = l > lo ? l : lo;
+        hi = h < hi ? h : hi;
+        if (lo > hi) break;
+        const char *vbase = sino + (size_t)j * viewbytes;
+        const uint32_t uoff = cv.uf * (uint32_t)Taps<DT>::kQuadBytes;
+        // batches of B independent loads, issued before any of them is used
 #pragma unroll
-        for (int k = 0; k < ZC; ++k) {
-            if (!(alive & (1u << k))) continue;
-            int vf;
-            float rv;
-            if (!v_index(a, cv.mag * zs[k], vmax, vf, rv)) {
-                alive &= ~(1u << k);
-                continue;
+        for (int k0 = 0; k0 < ZC; k0 += B) {
+            typename Taps<DT>::Raw raw[B];
+            float rv[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                float tk = fmaf((float)(k0 + i), dt, t0);
+                tk = __builtin_amdgcn_fmed3f(tk, 0.f, a.vlim);
+                uint32_t vf = (uint32_t)tk;
+                rv[i] = __builtin_amdgcn_fractf(tk);
+                raw[i] = Taps<DT>::load_raw(vbase + __umul24(vf, rowbytes) + uoff);
             }
-            float4 q = Taps<DT>::quad(a.sino, vbase + (size_t)vf * ncol);
-            float p0 = fmaf(cv.ru, q.y - q.x, q.x);
-            float p1 = fmaf(cv.ru, q.w - q.z, q.z);
-            acc[k] = fmaf(cv.coef, fmaf(rv, p1 - p0, p0), acc[k]);
+#pragma unroll
+            for (int i = 0; i < B; ++i)
+                acc[k0 + i] = fmaf(cv.coef, bilerp(Taps<DT>::cvt(raw[i]), cv.ru, rv[i]), acc[k0 + i]);
         }
     }
     const size_t plane = (size_t)a.nx * a.nx;
     float *out = a.vol + (size_t)iy * a.nx + ix;
 #pragma unroll
     for (int k = 0; k < ZC; ++k)
-        if (k < nk) out[(size_t)(z0 + k) * plane] = (alive & (1u << k)) ? acc[k] : 0.f;
+        if (k < nk) out[(size_t)(z0 + k) * plane] = (k >= lo && k <= hi) ? acc[k] : 0.f;
 }
 
 // Fan beam, axial view, PAIR taps; slice z reads detector row z.
@@ -231,7 +305,7 @@ __global__ __launch_bounds__(256) void bp_fan_axial_kernel(BpArgs a) {
         for (int k = 0; k < ZC; ++k) {
             if (k < nk) {
                 float2 p = Taps<DT>::pair(a.sino, base + (size_t)k * row_stride);
-                acc[k] = fmaf(cv.coef, fmaf(cv.ru, p.y - p.x, p.x), acc[k]);
+                acc[k] = fmaf(cv.coef, fmaf(cv.ru, p.y, p.x), acc[k]);
             }
         }
     }
@@ -324,24 +398,22 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
             }
             int vf = (int)vff;
             float rv = vidx - vff;
-            float p0, p1;
             if (a.layout == MCT_QUAD) {
                 float4 q = Taps<DT>::quad(a.sino, ((size_t)j * (a.nv - 1) + vf) * ncol + uf);
-                p0 = q.x * (1.f - ru) + q.y * ru;
-                p1 = q.z * (1.f - ru) + q.w * ru;
+                acc += coef * bilerp(q, ru, rv);
             } else {
                 size_t r0 = ((size_t)vf * a.nviews + j) * a.nu + uf;
                 size_t r1 = r0 + (size_t)a.nviews * a.nu;
-                p0 = Taps<DT>::one(a.sino, r0) * (1.f - ru) + Taps<DT>::one(a.sino, r0 + 1) * ru;
-                p1 = Taps<DT>::one(a.sino, r1) * (1.f - ru) + Taps<DT>::one(a.sino, r1 + 1) * ru;
+                float p0 = Taps<DT>::one(a.sino, r0) * (1.f - ru) + Taps<DT>::one(a.sino, r0 + 1) * ru;
+                float p1 = Taps<DT>::one(a.sino, r1) * (1.f - ru) + Taps<DT>::one(a.sino, r1 + 1) * ru;
+                acc += coef * ((1.f - rv) * p0 + rv * p1);
             }
-            acc += coef * ((1.f - rv) * p0 + rv * p1);
         } else {
             float s0, s1;
             if (a.layout == MCT_PAIR) {
                 float2 p = Taps<DT>::pair(a.sino, ((size_t)iz * a.nviews + j) * ncol + uf);
                 s0 = p.x;
-                s1 = p.y;
+                s1 = p.x + p.y;
             } else {
                 size_t r0 = ((size_t)iz * a.nviews + j) * a.nu + uf;
                 s0 = Taps<DT>::one(a.sino, r0);
@@ -466,6 +538,13 @@ BpArgs make_args(const mct_plan *p) {
     a.dv = p->dv;
     a.adv = p->adv;
     a.vinv = (g.cone && g.det_h > 0.f) ? (float)((double)p->adv / ((double)g.det_h * (double)p->dv)) : 0.f;
+    a.inv_du = (float)(1.0 / (double)p->du);
+    // largest float below n_v - 1: a clamped row coordinate floors to <= n_v - 2
+    a.vlim = g.n_v >= 2 ? nextafterf((float)(g.n_v - 1), 0.f) : 0.f;
+    a.ua = p->ua;
+    a.ub = p->ub;
+    a.da = p->da;
+    a.db = p->db;
     a.curved = g.curved;
     a.cone = g.cone;
     a.view_mode = g.view_mode;
@@ -506,23 +585,34 @@ int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t
     a.vol = vol;
     const bool fast = g.view_mode == 1 && !g.apply_pmatrix && layout != MCT_PLAIN;
     if (fast) {
-        int zc = ctx->bp_zc ? ctx->bp_zc : (g.cone ? 16 : 8);
+        int zc = ctx->bp_zc ? ctx->bp_zc : (g.cone ? 32 : 8);
         a.tiles_x = (g.img_dim + 15) / 16;
         a.tiles_y = a.tiles_x;
         a.nzc = (g.img_dim_z + zc - 1) / zc;
         int nblocks = a.tiles_x * a.tiles_y * a.nzc;
         if (g.cone) {
-#define CASE_DT(ZC)                                                                  \
-    if (dtype == MCT_F32) MCT_AXIAL_LAUNCH(bp_cone_axial_kernel, ZC, MCT_F32);       \
-    else if (dtype == MCT_F16) MCT_AXIAL_LAUNCH(bp_cone_axial_kernel, ZC, MCT_F16);  \
-    else MCT_AXIAL_LAUNCH(bp_cone_axial_kernel, ZC, MCT_BF16);
-            switch (zc) {
-                case 4: CASE_DT(4) break;
-                case 8: CASE_DT(8) break;
-                case 16: CASE_DT(16) break;
-                default: CASE_DT(32) break;
-            }
-#undef CASE_DT
+            int bt = ctx->bp_batch ? ctx->bp_batch : 8;
+            bt = bt > zc ? zc : bt;
+#define CONE(ZC, B)                                                                            \
+    if (dtype == MCT_F32)                                                                      \
+        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_F32>), dim3(nblocks), dim3(256), 0, \
+                           ctx->stream, a);                                                    \
+    else if (dtype == MCT_F16)                                                                 \
+        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_F16>), dim3(nblocks), dim3(256), 0, \
+                           ctx->stream, a);                                                    \
+    else                                                                                       \
+        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_BF16>), dim3(nblocks), dim3(256), 0, \
+                           ctx->stream, a);
+            if (zc == 4) { CONE(4, 4) }
+            else if (zc == 8 && bt == 4) { CONE(8, 4) }
+            else if (zc == 8) { CONE(8, 8) }
+            else if (zc == 16 && bt == 4) { CONE(16, 4) }
+            else if (zc == 16 && bt == 8) { CONE(16, 8) }
+            else if (zc == 16) { CONE(16, 16) }
+            else if (bt == 4) { CONE(32, 4) }
+            else if (bt == 8) { CONE(32, 8) }
+            else { CONE(32, 16) }
+#undef CONE
         } else {
 #define CASE_DT(ZC)                                                                  \
     if (dtype == MCT_F32) MCT_AXIAL_LAUNCH(bp_fan_axial_kernel, ZC, MCT_F32);        \

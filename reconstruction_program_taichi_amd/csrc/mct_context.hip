@@ -28,6 +28,55 @@ void free_plan_arrays(mct_plan *p) {
 // table is reproducible on any host.
 inline float cos_rn(float a) { return (float)std::cos((double)a); }
 inline float sin_rn(float a) { return (float)std::sin((double)a); }
+
+// Order-preserving map between floats and integers (for bisection over
+// every representable float).
+inline int64_t fkey(float f) {
+    int32_t i;
+    std::memcpy(&i, &f, 4);
+    return i >= 0 ? (int64_t)i : -(int64_t)(i & 0x7fffffff);
+}
+inline float funkey(int64_t k) {
+    uint32_t u = k >= 0 ? (uint32_t)k : ((uint32_t)(-k) | 0x80000000u);
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Smallest float in (bad, good] where pred holds, pred monotone along the
+// float order with pred(bad) false and pred(good) true (good > bad), or the
+// largest float in [good, bad) when good > bad.
+template <class P>
+float boundary(P pred, float good, float bad) {
+    int64_t g = fkey(good), b = fkey(bad);
+    while (g - b > 1 || b - g > 1) {
+        int64_t m = g + (b - g) / 2;
+        if (pred(funkey(m)))
+            g = m;
+        else
+            b = m;
+    }
+    return funkey(g);
+}
+
+// The reference's range tests in FP32, in its expression order.
+struct URange {
+    float u0, du;
+    int nu;
+    bool operator()(float u) const {
+        float uidx = (u - u0) / du;
+        return !(uidx < 0.f || uidx + 1.f > (float)(nu - 1));
+    }
+};
+struct VRange {
+    float v0, det_h, adv, dv;
+    int nv;
+    bool operator()(float d) const {
+        float vidx = d / det_h * adv / dv;
+        float vf = std::floor(vidx);
+        return !(vf < 0.f || vf + 1.f > (float)(nv - 1));
+    }
+};
 }  // namespace
 
 namespace mct {
@@ -138,11 +187,14 @@ int mct_synchronize(mct_ctx *ctx) {
     return MCT_OK;
 }
 
-int mct_set_bp_tile(mct_ctx *ctx, int32_t zc) {
+int mct_set_bp_tile(mct_ctx *ctx, int32_t zc, int32_t batch) {
     if (!ctx) return mct::set_error(MCT_ERR_INVALID, "ctx is NULL");
     if (!(zc == 0 || zc == 4 || zc == 8 || zc == 16 || zc == 32))
         return mct::set_error(MCT_ERR_INVALID, "zc must be 0, 4, 8, 16 or 32 (got %d)", zc);
+    if (!(batch == 0 || batch == 4 || batch == 8 || batch == 16))
+        return mct::set_error(MCT_ERR_INVALID, "batch must be 0, 4, 8 or 16 (got %d)", batch);
     ctx->bp_zc = zc;
+    ctx->bp_batch = batch;
     return MCT_OK;
 }
 
@@ -184,10 +236,11 @@ int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
         std::vector<float4> bt(nv);
         std::vector<float2> ft(nv);
         const float *a = g->angles;
+        const float divf = mct::div_factor_of(*g);
         for (int j = 0; j < nv; ++j) {
             float ar = a[j] - g->img_rot;
             float d = (j == nv - 1) ? std::fabs(a[nv - 1] - a[0]) / (float)(nv - 1) : std::fabs(a[j + 1] - a[j]);
-            bt[j] = make_float4(cos_rn(ar), sin_rn(ar), d, 0.f);
+            bt[j] = make_float4(cos_rn(ar), sin_rn(ar), d, g->sid * d * divf);
             ft[j] = make_float2(cos_rn(a[j]), sin_rn(a[j]));
         }
         rc = upload(&p->bp_view, bt.data(), nv);
@@ -212,6 +265,29 @@ int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
     p->div_factor = mct::div_factor_of(*g);
     p->fp_count_mode = g->fpj_count_mode;
     p->zpv = mct::helical_z_per_view(p, g->angles);
+    {
+        URange ur{p->u0, p->du, g->n_u};
+        float umid = p->u0 + p->du * (float)(g->n_u - 1) * 0.5f;
+        if (ur(umid)) {
+            p->ua = boundary(ur, umid, -3e38f);
+            p->ub = boundary(ur, umid, 3e38f);
+        } else {  // no u passes (n_u == 2 with a degenerate spacing): empty interval
+            p->ua = 1.f;
+            p->ub = -1.f;
+        }
+        if (g->cone && g->n_v >= 2 && g->det_h > 0.f) {
+            VRange vr{p->v0, g->det_h, p->adv, p->dv, g->n_v};
+            float dmid = (float)(g->n_v - 1) * 0.5f * g->det_h * (p->dv > 0.f ? 1.f : -1.f);
+            if (vr(dmid)) {
+                float e0 = boundary(vr, dmid, -3e38f), e1 = boundary(vr, dmid, 3e38f);
+                p->da = e0 < e1 ? e0 : e1;
+                p->db = e0 < e1 ? e1 : e0;
+            } else {
+                p->da = 1.f;
+                p->db = -1.f;
+            }
+        }
+    }
     *out = p;
     return MCT_OK;
 }

@@ -12,7 +12,8 @@ struct mct_ctx {
     hipStream_t stream = nullptr;
     int cu_count = 256;
     int xcd_count = 8;
-    int bp_zc = 0;  // 0: default slices per thread in the axial back-projector
+    int bp_zc = 0;     // slices per thread in the axial back-projector (0: default)
+    int bp_batch = 0;  // loads issued together by the cone back-projector (0: default)
     // partial sums of the deterministic FP64 reductions
     double *partials = nullptr;
     int64_t partials_cap = 0;
@@ -35,7 +36,7 @@ struct mct_plan {
     float *angles = nullptr, *u = nullptr, *v = nullptr;
     float *pmatrix = nullptr, *center_z = nullptr, *matrix_a = nullptr, *x_s = nullptr;
     // per-view tables
-    float4 *bp_view = nullptr;  // {cos(b_j - rot), sin(b_j - rot), dbeta_j, 0}
+    float4 *bp_view = nullptr;  // {cos(b_j - rot), sin(b_j - rot), dbeta_j, sid*dbeta_j*div}
     float2 *fp_view = nullptr;  // {cos b_j, sin b_j}
     // host-side scalars derived once
     float u0 = 0.f, du = 1.f;           // u[0], u[1] - u[0]
@@ -45,6 +46,10 @@ struct mct_plan {
     float cos_a0 = 1.f, sin_a0 = 0.f;  // of angles[0] (P-matrix forward projection)
     float zpv = 0.f;                   // helical z shift per view (forward projection)
     int fp_count_mode = 0;
+    // Exact detector-range tests of run_mgfbp.py:1142-1144 and 1170-1173 as
+    // intervals: the u test passes iff ua <= u <= ub for u = fl(mag * yr);
+    // the v test passes iff da <= d <= db for d = fl(fl(mag * z) - v[0]).
+    float ua = 0.f, ub = 0.f, da = 0.f, db = 0.f;
 };
 
 namespace mct {

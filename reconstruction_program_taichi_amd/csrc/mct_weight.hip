@@ -19,12 +19,11 @@ __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int n
                               const float *__restrict__ uarr, const float *__restrict__ varr,
                               const float *__restrict__ ang, float sdd, float scan,
                               int short_scan, int curved, BHCoef bh, int bh_order) {
-    int64_t n = (int64_t)nv * nviews * nu;
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    int j = (int)(idx % nu);
-    int i = (int)((idx / nu) % nviews);
-    int s = (int)(idx / ((int64_t)nu * nviews));
+    // grid: x over elements j, y over views i, z over rows s
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, s = blockIdx.z;
+    if (j >= nu) return;
+    const size_t idx = ((size_t)s * nviews + i) * nu + j;
     float p = sgm[idx];
     if (bh_order > 0) {
         float acc = 0.f, pw = p;
@@ -97,50 +96,52 @@ template <int DT> __device__ __forceinline__ uint16_t to_half_bits(float f) {
     return DT == MCT_F16 ? mct::f32_to_f16_rn(f) : mct::f32_to_bf16_rn(f);
 }
 
-// QUAD: out[j][r][c] = {in[r][j][c], in[r][j][c+1], in[r+1][j][c], in[r+1][j][c+1]}
+// QUAD: out[j][r][c] = bilinear coefficients of cell (r, c) of view j:
+// {a, b - a, c - a, d - c - b + a} with a = in[r][j][c], b = in[r][j][c+1],
+// c = in[r+1][j][c], d = in[r+1][j][c+1] (each rounded once from FP64)
 template <int DT>
 __global__ void pack_quad_kernel(const float *__restrict__ in, int rows, int views, int nu,
                                  void *__restrict__ out) {
-    int nc = nu - 1, nr = rows - 1;
-    int64_t n = (int64_t)views * nr * nc;
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    int c = (int)(idx % nc);
-    int r = (int)((idx / nc) % nr);
-    int j = (int)(idx / ((int64_t)nc * nr));
+    // grid: x over detector cells c, y over rows r, z over views j
+    const int nc = nu - 1, nr = rows - 1;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y, j = blockIdx.z;
+    if (c >= nc) return;
     const float *p0 = in + ((int64_t)r * views + j) * nu + c;
     const float *p1 = p0 + (int64_t)views * nu;
-    float a = p0[0], b = p0[1], cc = p1[0], d = p1[1];
+    const double a = p0[0], b = p0[1], cc = p1[0], d = p1[1];
+    // bilinear coefficients: v(ru, rv) = q0 + ru q1 + rv (q2 + ru q3)
+    const float q0 = (float)a, q1 = (float)(b - a), q2 = (float)(cc - a), q3 = (float)(d - cc - b + a);
+    const size_t idx = ((size_t)j * nr + r) * nc + c;
     if (DT == MCT_F32) {
-        reinterpret_cast<float4 *>(out)[idx] = make_float4(a, b, cc, d);
+        reinterpret_cast<float4 *>(out)[idx] = make_float4(q0, q1, q2, q3);
     } else {
         ushort4 h;
-        h.x = to_half_bits<DT>(a);
-        h.y = to_half_bits<DT>(b);
-        h.z = to_half_bits<DT>(cc);
-        h.w = to_half_bits<DT>(d);
+        h.x = to_half_bits<DT>(q0);
+        h.y = to_half_bits<DT>(q1);
+        h.z = to_half_bits<DT>(q2);
+        h.w = to_half_bits<DT>(q3);
         reinterpret_cast<ushort4 *>(out)[idx] = h;
     }
 }
 
-// PAIR: out[r][j][c] = {in[r][j][c], in[r][j][c+1]}
+// PAIR: out[r][j][c] = {in[r][j][c], in[r][j][c+1] - in[r][j][c]}
 template <int DT>
 __global__ void pack_pair_kernel(const float *__restrict__ in, int rows, int views, int nu,
                                  void *__restrict__ out) {
-    int nc = nu - 1;
-    int64_t n = (int64_t)rows * views * nc;
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    int64_t rowbase = idx / nc;  // r*views + j
-    int c = (int)(idx % nc);
-    const float *p = in + rowbase * nu + c;
-    float a = p[0], b = p[1];
+    const int nc = nu - 1;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y, j = blockIdx.z;
+    if (c >= nc) return;
+    const float *p = in + ((int64_t)r * views + j) * nu + c;
+    const float q0 = p[0], q1 = (float)((double)p[1] - (double)p[0]);
+    const size_t idx = ((size_t)r * views + j) * nc + c;
     if (DT == MCT_F32) {
-        reinterpret_cast<float2 *>(out)[idx] = make_float2(a, b);
+        reinterpret_cast<float2 *>(out)[idx] = make_float2(q0, q1);
     } else {
         ushort2 h;
-        h.x = to_half_bits<DT>(a);
-        h.y = to_half_bits<DT>(b);
+        h.x = to_half_bits<DT>(q0);
+        h.y = to_half_bits<DT>(q1);
         reinterpret_cast<ushort2 *>(out)[idx] = h;
     }
 }
@@ -167,8 +168,9 @@ int mct_weight_sinogram(mct_plan *plan, float *sgm, const float *bh_coef, int32_
     mct_ctx *ctx = plan->ctx;
     BHCoef c = {};
     for (int t = 0; t < bh_order; ++t) c.c[t] = bh_coef[t];
-    int64_t n = (int64_t)g.n_v * g.n_views * g.n_u;
-    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+    MCT_CHECK(g.n_v <= 65535 && g.n_views <= 65535, "rows and views must be <= 65535");
+    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for(g.n_u, 256), g.n_views, g.n_v), dim3(256), 0,
+                       ctx->stream,
                        sgm, g.n_v, g.n_views, g.n_u, plan->u, plan->v, plan->angles, g.sdd,
                        g.total_scan_angle, g.short_scan, g.curved, c, bh_order);
     MCT_LAUNCHED();
@@ -213,33 +215,28 @@ int mct_pack_sinogram(mct_ctx *ctx, const float *in, int32_t rows, int32_t views
     int64_t bytes = mct_packed_bytes(rows, views, n_u, dtype, layout);
     MCT_CHECK(bytes > 0, "bad packed shape (rows %d, views %d, n_u %d, layout %d)", rows, views,
               n_u, layout);
-    int64_t n;
-#define MCT_PACK_LAUNCH(K)                                                                  \
-    do {                                                                                    \
-        if (dtype == MCT_F32)                                                               \
-            hipLaunchKernelGGL(K<MCT_F32>, dim3(mct::blocks_for(n, 256)), dim3(256), 0,     \
-                               ctx->stream, ARGS);                                          \
-        else if (dtype == MCT_F16)                                                          \
-            hipLaunchKernelGGL(K<MCT_F16>, dim3(mct::blocks_for(n, 256)), dim3(256), 0,     \
-                               ctx->stream, ARGS);                                          \
-        else                                                                                \
-            hipLaunchKernelGGL(K<MCT_BF16>, dim3(mct::blocks_for(n, 256)), dim3(256), 0,    \
-                               ctx->stream, ARGS);                                          \
+    MCT_CHECK(rows <= 65535 && views <= 65535, "rows and views must be <= 65535");
+    dim3 grid;
+    if (layout == MCT_QUAD)
+        grid = dim3(mct::blocks_for(n_u - 1, 256), rows - 1, views);
+    else if (layout == MCT_PAIR)
+        grid = dim3(mct::blocks_for(n_u - 1, 256), rows, views);
+    int64_t n = (int64_t)rows * views * n_u;
+#define MCT_PACK_LAUNCH(K, G, ...)                                                            \
+    do {                                                                                      \
+        if (dtype == MCT_F32)                                                                 \
+            hipLaunchKernelGGL(K<MCT_F32>, G, dim3(256), 0, ctx->stream, __VA_ARGS__);        \
+        else if (dtype == MCT_F16)                                                            \
+            hipLaunchKernelGGL(K<MCT_F16>, G, dim3(256), 0, ctx->stream, __VA_ARGS__);        \
+        else                                                                                  \
+            hipLaunchKernelGGL(K<MCT_BF16>, G, dim3(256), 0, ctx->stream, __VA_ARGS__);       \
     } while (0)
-    if (layout == MCT_QUAD) {
-        n = (int64_t)views * (rows - 1) * (n_u - 1);
-#define ARGS in, rows, views, n_u, out
-        MCT_PACK_LAUNCH(pack_quad_kernel);
-    } else if (layout == MCT_PAIR) {
-        n = (int64_t)rows * views * (n_u - 1);
-        MCT_PACK_LAUNCH(pack_pair_kernel);
-#undef ARGS
-    } else {
-        n = (int64_t)rows * views * n_u;
-#define ARGS in, n, out
-        MCT_PACK_LAUNCH(pack_plain_kernel);
-#undef ARGS
-    }
+    if (layout == MCT_QUAD)
+        MCT_PACK_LAUNCH(pack_quad_kernel, grid, in, rows, views, n_u, out);
+    else if (layout == MCT_PAIR)
+        MCT_PACK_LAUNCH(pack_pair_kernel, grid, in, rows, views, n_u, out);
+    else
+        MCT_PACK_LAUNCH(pack_plain_kernel, dim3(mct::blocks_for(n, 256)), in, n, out);
 #undef MCT_PACK_LAUNCH
     MCT_LAUNCHED();
     return MCT_OK;

@@ -61,8 +61,8 @@ class Context:
     def synchronize(self) -> None:
         call("mct_synchronize", self.h)
 
-    def set_bp_tile(self, zc: int) -> None:
-        call("mct_set_bp_tile", self.h, int(zc))
+    def set_bp_tile(self, zc: int, batch: int = 0) -> None:
+        call("mct_set_bp_tile", self.h, int(zc), int(batch))
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -113,14 +113,15 @@ def test_bp_fan_plain_generic(ctx):
     assert rel_l2(got, ref) <= 1e-5
 
 
-@pytest.mark.parametrize("zc", [4, 8, 16, 32])
-def test_bp_cone_quad_tiles(ctx, zc):
-    spec = cone_case()
-    ctx.set_bp_tile(zc)
+@pytest.mark.parametrize("zc,batch", [(4, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 4), (32, 8),
+                                      (32, 16)])
+def test_bp_cone_quad_tiles(ctx, zc, batch):
+    spec = cone_case(nz=37)  # taller than the detector's cone: top/bottom slices masked
+    ctx.set_bp_tile(zc, batch)
     try:
         got, ref, alive, _ = _bp_check(ctx, spec, "quad")
     finally:
-        ctx.set_bp_tile(0)
+        ctx.set_bp_tile(0, 0)
     assert 0.02 < (~alive).mean() < 0.98  # the detector-range mask is exercised
     assert mask_disagreements(got, ref, alive) == 0
     assert rel_l2(got, ref) <= 1e-5

@@ -1,0 +1,157 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+symbol include/mct.h declares (no GPU call), the JSONC loader and config
+parsing behave as the reference's, and the geometry arrays follow their
+closed forms."""
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from reconstruction_program_taichi_amd import _abi, config, geometry as geo
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_functions():
+    text = (ROOT / "include" / "mct.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mct_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    names = header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _abi.SIGNATURES, n
+    assert set(_abi.SIGNATURES) == set(names)
+    assert lib.mct_abi_version() == _abi.ABI_VERSION
+
+
+def test_packed_bytes_host_helper():
+    lib = _abi.load()
+    assert lib.mct_packed_bytes(256, 1080, 1024, _abi.MCT_F32, _abi.MCT_QUAD) == 255 * 1080 * 1023 * 16
+    assert lib.mct_packed_bytes(1, 720, 800, _abi.MCT_F16, _abi.MCT_PAIR) == 720 * 799 * 4
+    assert lib.mct_packed_bytes(3, 10, 8, _abi.MCT_BF16, _abi.MCT_PLAIN) == 3 * 10 * 8 * 2
+    assert lib.mct_packed_bytes(1, 10, 8, _abi.MCT_F32, _abi.MCT_QUAD) == -1
+    assert lib.mct_packed_bytes(0, 10, 8, _abi.MCT_F32, _abi.MCT_PLAIN) == -1
+
+
+def test_null_arguments_return_errors_not_crashes():
+    lib = _abi.load()
+    assert lib.mct_create(0, None) == _abi.MCT_ERR_INVALID
+    assert b"NULL" in lib.mct_last_error()
+    assert lib.mct_plan_create(None, None, None) == _abi.MCT_ERR_INVALID
+    assert lib.mct_set_bp_tile(None, 8, 0) == _abi.MCT_ERR_INVALID
+    assert lib.mct_destroy(None) == 0
+
+
+# ------------------------------------------------------------------ JSONC
+def test_remove_comments_matches_reference_regex():
+    s = '{\n// line\n"a": 1, /* block\n spans */ "b": "x//y"\n}'
+    out = config.remove_comments(s)
+    assert "line" not in out and "spans" not in out
+    # the reference's regex also strips // inside strings
+    assert '"x' in out and "y\"" not in out
+
+
+def test_load_jsonc(tmp_path):
+    p = tmp_path / "c.jsonc"
+    p.write_text('// header\n{\n  "Views": 720, // views\n  /* c */ "ConeBeam": false\n}\n')
+    assert config.ReadConfigFile(str(p)) == {"Views": 720, "ConeBeam": False}
+    out = tmp_path / "o.jsonc"
+    config.save_jsonc(str(out), {"a": [1, 2]})
+    assert json.loads(out.read_text()) == {"a": [1, 2]}
+    with pytest.raises(AssertionError):
+        config.save_jsonc(str(tmp_path / "o.json"), {})
+
+
+def test_imread_raw_offset_gap(tmp_path):
+    a = np.arange(5 * 4 * 3, dtype=np.float32).reshape(5, 4, 3)
+    p = tmp_path / "a.raw"
+    a.tofile(p)
+    r = config.imreadRaw(str(p), height=2, width=3, nSlice=5, offset=3 * 4, gap=2 * 3 * 4)
+    assert np.array_equal(r, a[:, 1:3, :])
+
+
+def _fbp_cfg(tmp_path, **kw):
+    c = {"InputDir": str(tmp_path), "OutputDir": str(tmp_path / "out"), "InputFiles": "sgm_.*",
+         "OutputFilePrefix": "", "OutputFileReplace": ["sgm", "rec"],
+         "DetectorElementCountHorizontal": 100, "DetectorElementWidth": 0.8,
+         "DetectorOffsetHorizontal": 0.5, "DetectorElementCountVertical": 2, "Views": 90,
+         "SourceIsocenterDistance": 250, "SourceDetectorDistance": 500, "ImageDimension": 64,
+         "PixelSize": 0.6, "HammingFilter": 1}
+    c.update(kw)
+    return c
+
+
+def test_mgfbp_config_defaults_and_arrays(tmp_path):
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    m = Mgfbp(_fbp_cfg(tmp_path, ImageRotation=10.0))
+    assert m.short_scan == 0 and m.img_dim_z == 2 and m.output_file_format == "raw"
+    spec = m.geometry_spec()
+    i = np.arange(100)
+    assert np.allclose(spec.u, -(i - 49.5) * 0.8 - 0.5, atol=1e-5)
+    assert np.allclose(spec.angles, 2 * geo.PI / 90 * np.arange(90) + 10 / 180 * geo.PI, atol=1e-6)
+    assert np.all(spec.v == 0)
+    assert spec.u.dtype == np.float32 and spec.angles.dtype == np.float32
+
+
+def test_mgfbp_cone_center_z_default(tmp_path):
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    m = Mgfbp(_fbp_cfg(tmp_path, ConeBeam=True, DetectorElementHeight=0.4, ImageDimensionZ=8,
+                       VoxelHeight=0.3, DetectorElementCountVertical=20, DetectorOffsetVertical=1.0,
+                       DetectorElementVerticalReconRange=[4, 11]))
+    # run_mgfbp.py:651-656: centre row of the range, magnified back to the isocentre
+    expect = (1.0 + ((4 + 11) / 2 - 19 / 2) * 0.4) * 250 / 500
+    assert abs(m.img_center_z - expect) < 1e-12
+    spec = m.geometry_spec()
+    assert spec.n_v == 8
+    assert np.allclose(spec.v, (np.arange(8) + 4 - 9.5) * 0.4 + 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    ({"ImageDimension": -3}, "ImageDimension must be a positive integer"),
+    ({"OutputFileFormat": "png"}, "Output file format"),
+    ({"ReconViewMode": "Coronal"}, "ReconViewMode can only be"),
+    ({"DetectorElementVerticalReconRange": [500, 700]}, "Out of detector row range"),
+    ({"ConeBeam": "yes"}, "ConeBeam must be true or false"),
+])
+def test_mgfbp_config_errors_exit(tmp_path, capsys, kw, msg):
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    with pytest.raises(SystemExit):
+        Mgfbp(_fbp_cfg(tmp_path, **kw))
+    assert msg in capsys.readouterr().out
+
+
+def test_missing_horizontal_offset_raises_attribute_error(tmp_path):
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    c = _fbp_cfg(tmp_path)
+    del c["DetectorOffsetHorizontal"]
+    with pytest.raises(AttributeError):
+        Mgfbp(c)
+
+
+def test_missing_filter_key_fails_at_filtering(tmp_path):
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    c = _fbp_cfg(tmp_path)
+    del c["HammingFilter"]
+    m = Mgfbp(c)
+    with pytest.raises(AttributeError):
+        m.recon_kernel()
+
+
+def test_div_factor_cases():
+    assert geo.div_factor(2 * geo.PI, False) == pytest.approx(0.5)
+    assert geo.div_factor(4 * geo.PI, False) == pytest.approx(0.25)
+    assert geo.div_factor(geo.PI + 0.3, True) == 1.0
+    assert geo.div_factor(2 * geo.PI + 1.0, True) == pytest.approx(0.5)
+    assert geo.div_factor(2 * geo.PI + 4.0, True) == pytest.approx(1 / 3)
