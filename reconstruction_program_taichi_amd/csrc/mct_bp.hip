@@ -34,6 +34,10 @@
 
 namespace {
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (gfx950 included):
+// 32-bit untyped data, no swizzle.
+constexpr int kBufferDword3 = 0x00020000;
+
 struct BpArgs {
     int nx, nz, nu, nv, nviews;
     float pix, vh, cx, cy, cz, cos_rot, sin_rot;
@@ -57,6 +61,9 @@ template <> struct Taps<MCT_F32> {
     __device__ __forceinline__ static Raw load_raw(const char *p) {
         return *reinterpret_cast<const float4 *>(p);
     }
+    __device__ __forceinline__ static Raw load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    }
     __device__ __forceinline__ static float4 cvt(float4 q) { return q; }
     __device__ __forceinline__ static float4 quad(const void *base, size_t i) {
         return reinterpret_cast<const float4 *>(base)[i];
@@ -76,6 +83,9 @@ template <> struct Taps<MCT_F16> {
     using Raw = ushort4;
     __device__ __forceinline__ static Raw load_raw(const char *p) {
         return *reinterpret_cast<const ushort4 *>(p);
+    }
+    __device__ __forceinline__ static Raw load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_bit_cast(ushort4, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
     __device__ __forceinline__ static float4 cvt(ushort4 h) {
         return make_float4(mct::f16_to_f32(h.x), mct::f16_to_f32(h.y), mct::f16_to_f32(h.z),
@@ -100,6 +110,9 @@ template <> struct Taps<MCT_BF16> {
     using Raw = uint2;
     __device__ __forceinline__ static Raw load_raw(const char *p) {
         return *reinterpret_cast<const uint2 *>(p);
+    }
+    __device__ __forceinline__ static Raw load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
     __device__ __forceinline__ static float4 cvt(uint2 w) {
         return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u),
@@ -193,7 +206,8 @@ __device__ __forceinline__ bool v_exact(const BpArgs &a, float mag, int iz, floa
 __device__ __forceinline__ void v_interval(const BpArgs &a, float mag, int z0, int nk, float halfz,
                                            float t0, float dt, int &lo, int &hi) {
     const float vmax = (float)(a.nv - 1);
-    float k_at0 = -t0 / dt, k_atm = (vmax - t0) / dt;
+    const float rdt = __builtin_amdgcn_rcpf(dt);  // approximate: the ends are confirmed below
+    float k_at0 = -t0 * rdt, k_atm = (vmax - t0) * rdt;
     float kf_lo = dt > 0.f ? k_at0 : k_atm, kf_hi = dt > 0.f ? k_atm : k_at0;
     // first k with t in range, last k with t in range (fast)
     kf_lo = fminf(fmaxf(kf_lo, -2.f), (float)nk + 2.f);
@@ -250,7 +264,9 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
         lo = l > lo ? l : lo;
         hi = h < hi ? h : hi;
         if (lo > hi) break;
-        const char *vbase = sino + (size_t)j * viewbytes;
+        // one buffer resource per view (SGPRs): 32-bit per-lane offsets
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(sino + (size_t)j * viewbytes), 0, (int)viewbytes, kBufferDword3);
         const uint32_t uoff = cv.uf * (uint32_t)Taps<DT>::kQuadBytes;
         // batches of B independent loads, issued before any of them is used
 #pragma unroll
@@ -263,7 +279,7 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
                 tk = __builtin_amdgcn_fmed3f(tk, 0.f, a.vlim);
                 uint32_t vf = (uint32_t)tk;
                 rv[i] = __builtin_amdgcn_fractf(tk);
-                raw[i] = Taps<DT>::load_raw(vbase + __umul24(vf, rowbytes) + uoff);
+                raw[i] = Taps<DT>::load_buf(rsrc, __umul24(vf, rowbytes) + uoff);
             }
 #pragma unroll
             for (int i = 0; i < B; ++i)
@@ -585,13 +601,13 @@ int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t
     a.vol = vol;
     const bool fast = g.view_mode == 1 && !g.apply_pmatrix && layout != MCT_PLAIN;
     if (fast) {
-        int zc = ctx->bp_zc ? ctx->bp_zc : (g.cone ? 32 : 8);
+        int zc = ctx->bp_zc ? ctx->bp_zc : (g.cone ? 16 : 8);
         a.tiles_x = (g.img_dim + 15) / 16;
         a.tiles_y = a.tiles_x;
         a.nzc = (g.img_dim_z + zc - 1) / zc;
         int nblocks = a.tiles_x * a.tiles_y * a.nzc;
         if (g.cone) {
-            int bt = ctx->bp_batch ? ctx->bp_batch : 8;
+            int bt = ctx->bp_batch ? ctx->bp_batch : 16;
             bt = bt > zc ? zc : bt;
 #define CONE(ZC, B)                                                                            \
     if (dtype == MCT_F32)                                                                      \
