@@ -74,6 +74,23 @@ def test_filter_rows_f32(ctx, n, rows):
     assert rel_l2(got, ref64) <= 1e-5
 
 
+@pytest.mark.parametrize("method,tol", [("bf16", 1e-2), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("n,rows", [(1024, 300), (800, 129), (100, 7), (37, 1)])
+def test_filter_rows_mfma(ctx, method, tol, n, rows):
+    """Toeplitz GEMM on MFMA against the FP64 oracle: BF16 operands (relative
+    rounding 2^-9 per operand) and split-BF16 (about 2^-17)."""
+    rng = np.random.default_rng(n + rows)
+    x = rng.standard_normal((rows, n)).astype(np.float32)
+    taps = recon.ramlak_taps(n, 0.4, np.float32)
+    out = torch.empty((rows, n), dtype=torch.float32, device=DEV)
+    ctx.filter_rows(dev(x), out, taps, 0.4, method)
+    got = host(out)
+    ref64 = recon.filter_rows(x.astype(np.float64), taps.astype(np.float64), 0.4, np.float64)
+    err = rel_l2(got, ref64)
+    print(f"{method} n={n}: rel-L2 vs FP64 {err:.2e}")
+    assert err <= tol
+
+
 def test_filter_rows_empty(ctx):
     out = torch.empty((0, 16), dtype=torch.float32, device=DEV)
     inp = torch.empty((0, 16), dtype=torch.float32, device=DEV)
