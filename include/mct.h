@@ -214,15 +214,18 @@ int mct_dot_f64(mct_ctx *ctx, const float *a, const float *b, int64_t n, double 
 /* Mgfbp_ir.GenerateWR run_mgfbp_iterative_recon_IRN_CGLS.py:234-243:
  * wr = 1/sqrt(ux^2 + uy^2 + uz^2 + beta) of y = x - prior (prior may be NULL),
  * with forward differences along axis 1 (ux), axis 2 (uy) and axis 0 (uz,
- * only when nz > 2); the last index of each axis has difference 0.
+ * only when couple_z != 0 and nz > 2 - the reference couples slices exactly
+ * then; couple_z = 0 treats the slices as independent images); the last
+ * index of each axis has difference 0.
  * Evaluated in FP64, stored FP32.  All arrays DEVICE [nz][ny][nx]. */
 int mct_tv_weight(mct_ctx *ctx, const float *x, const float *prior, int32_t nz, int32_t ny,
-                  int32_t nx, double beta, float *wr);
+                  int32_t nx, int32_t couple_z, double beta, float *wr);
 
 /* Mgfbp_ir.Dt_W_D run_mgfbp_iterative_recon_IRN_CGLS.py:267-286 scaled:
- * out = base + scale * Dt W D x (base may be NULL = 0; may alias out). */
+ * out = base + scale * Dt W D x (base may be NULL = 0; may alias out;
+ * couple_z as for mct_tv_weight). */
 int mct_dtwd(mct_ctx *ctx, const float *x, const float *wr, int32_t nz, int32_t ny, int32_t nx,
-             float scale, const float *base, float *out);
+             int32_t couple_z, float scale, const float *base, float *out);
 
 /* out = a*x + b*y elementwise, FP32 (a, b host scalars). */
 int mct_axpby(mct_ctx *ctx, float a, const float *x, float b, const float *y, float *out,
@@ -246,7 +249,7 @@ int mct_cg_direction(mct_ctx *ctx, float *d, const float *r, double *s, int64_t 
 int mct_sq_diff_f64(mct_ctx *ctx, const float *a, const float *b, const float *mb, int64_t n,
                     double *out);
 int mct_tv_value(mct_ctx *ctx, const float *x, const float *prior, int32_t nz, int32_t ny,
-                 int32_t nx, double beta, double *out);
+                 int32_t nx, int32_t couple_z, double beta, double *out);
 
 #ifdef __cplusplus
 }

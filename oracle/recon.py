@@ -588,3 +588,53 @@ def cg_solve(apply_A, b, x0, iters):
         beta = float(np.sum(r * r)) / rr
         d = r + beta * d
     return x
+
+
+def ir_solve(sgm, g, x0, lam, beta_tv, n_irn, n_iter, couple_z=True, prior=None, alpha=0.0,
+             dtype=np.float64):
+    """Mgfbp_ir.MainFunction / TikhonovSol (run_mgfbp_iterative_recon_IRN_CGLS.py:
+    157-232) and, with a prior, Mgfbp_ir_piccs (…_piccs.py:83-150), with the
+    oracle's projectors: ray map, P^T b, then 1 + n_irn reweighted CG passes of
+    n_iter iterations.  couple_z=False drops the z difference (independent
+    slices).  Returns x."""
+    sgm = np.asarray(sgm, dtype=dtype)
+    rmask = ray_mask(g, dtype=dtype)
+    nz = x0.shape[0]
+    zc = couple_z and nz > 2
+
+    def tvw(x):
+        if zc or nz <= 2:
+            return tv_weight(x, beta_tv)
+        return np.stack([tv_weight(x[k:k + 1], beta_tv)[0] for k in range(nz)])
+
+    def dtwd_(x, w):
+        if zc or nz <= 2:
+            return dtwd(x, w)
+        return np.stack([dtwd(x[k:k + 1], w[k:k + 1])[0] for k in range(nz)])
+
+    def fpbp(x):
+        fp = forward_project(x, g, ray_mask=rmask, dtype=dtype)
+        return backproject_ir(fp, g, dtype=dtype).astype(np.float64)
+
+    ratio = sgm.size / x0.size
+    bpb = backproject_ir(sgm, g, ray_mask=rmask, dtype=dtype).astype(np.float64)
+    x = np.asarray(x0 if prior is None else prior, dtype=np.float64)
+    c1 = lam * (1 - alpha) * ratio
+    c2 = lam * alpha * ratio
+    w = np.ones_like(x)
+    wp = np.ones_like(x)
+    for p in range(1 + n_irn):
+        if p > 0:
+            w = tvw(x)
+            if prior is not None:
+                wp = tvw(x - prior)
+
+        def A(v):
+            out = fpbp(v) + c1 * dtwd_(v, w)
+            if prior is not None:
+                out = out + c2 * dtwd_(v, wp)
+            return out
+
+        b = bpb + (c2 * dtwd_(np.asarray(prior, np.float64), wp) if prior is not None else 0.0)
+        x = cg_solve(A, b, x, n_iter)
+    return x

@@ -78,13 +78,13 @@ SIGNATURES = {
     "mct_add_noise": (C.c_int, [_P, _P, _I64, _F32, _U64]),
     "mct_backproject_ir": (C.c_int, [_P, _P, _P, _P]),
     "mct_dot_f64": (C.c_int, [_P, _P, _P, _I64, _P]),
-    "mct_tv_weight": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _F64, _P]),
-    "mct_dtwd": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _F32, _P, _P]),
+    "mct_tv_weight": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _F64, _P]),
+    "mct_dtwd": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _F32, _P, _P]),
     "mct_axpby": (C.c_int, [_P, _F32, _P, _F32, _P, _P, _I64]),
     "mct_cg_update": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64]),
     "mct_cg_direction": (C.c_int, [_P, _P, _P, _P, _I64]),
     "mct_sq_diff_f64": (C.c_int, [_P, _P, _P, _P, _I64, _P]),
-    "mct_tv_value": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _F64, _P]),
+    "mct_tv_value": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _F64, _P]),
 }
 
 _lib = None

@@ -61,7 +61,8 @@ __global__ __launch_bounds__(kThreads) void sq_diff_kernel(const float *__restri
 }
 
 // Forward differences of y = x - prior at (z, yy, xx): ux along axis 1, uy
-// along axis 2, uz along axis 0 (only when nz > 2); 0 at the last index.
+// along axis 2, uz along axis 0 (only when cz, i.e. slices are coupled and
+// nz > 2); 0 at the last index.
 struct Grad {
     double ux, uy, uz;
 };
@@ -69,37 +70,37 @@ __device__ __forceinline__ double val(const float *x, const float *prior, int64_
     return prior ? (double)x[i] - (double)prior[i] : (double)x[i];
 }
 __device__ __forceinline__ Grad grad_at(const float *x, const float *prior, int nz, int ny, int nx,
-                                        int z, int yy, int xx) {
+                                        int cz, int z, int yy, int xx) {
     int64_t i = ((int64_t)z * ny + yy) * nx + xx;
     double c = val(x, prior, i);
     Grad g;
     g.ux = yy + 1 < ny ? val(x, prior, i + nx) - c : 0.0;
     g.uy = xx + 1 < nx ? val(x, prior, i + 1) - c : 0.0;
-    g.uz = (nz > 2 && z + 1 < nz) ? val(x, prior, i + (int64_t)ny * nx) - c : 0.0;
+    g.uz = (cz && z + 1 < nz) ? val(x, prior, i + (int64_t)ny * nx) - c : 0.0;
     return g;
 }
 
 __global__ __launch_bounds__(kThreads) void tv_weight_kernel(const float *__restrict__ x,
                                                              const float *__restrict__ prior, int nz,
-                                                             int ny, int nx, double beta,
+                                                             int ny, int nx, int cz, double beta,
                                                              float *__restrict__ wr) {
     int64_t n = (int64_t)nz * ny * nx;
     int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     int xx = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
-    Grad g = grad_at(x, prior, nz, ny, nx, z, yy, xx);
+    Grad g = grad_at(x, prior, nz, ny, nx, cz, z, yy, xx);
     wr[i] = (float)(1.0 / sqrt(g.ux * g.ux + g.uy * g.uy + g.uz * g.uz + beta));
 }
 
 __global__ __launch_bounds__(kThreads) void tv_value_kernel(const float *__restrict__ x,
                                                             const float *__restrict__ prior, int nz,
-                                                            int ny, int nx, double beta,
+                                                            int ny, int nx, int cz, double beta,
                                                             double *__restrict__ partials) {
     int64_t n = (int64_t)nz * ny * nx;
     double v = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)kReduceBlocks * kThreads) {
         int xx = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
-        Grad g = grad_at(x, prior, nz, ny, nx, z, yy, xx);
+        Grad g = grad_at(x, prior, nz, ny, nx, cz, z, yy, xx);
         v += sqrt(g.ux * g.ux + g.uy * g.uy + g.uz * g.uz + beta);
     }
     v = block_sum(v);
@@ -109,10 +110,10 @@ __global__ __launch_bounds__(kThreads) void tv_value_kernel(const float *__restr
 // Dt_W_D (run_mgfbp_iterative_recon_IRN_CGLS.py:267-286): with the weighted
 // differences gy = W*ux, gx = W*uy, gz = W*uz,
 //   out = -(gy[y] - gy[y-1]) [1 <= y <= ny-2] - (gx[x] - gx[x-1]) [1 <= x <= nx-2]
-//         - (gz[z] - gz[z-1]) [1 <= z <= nz-2, nz > 2].
+//         - (gz[z] - gz[z-1]) [1 <= z <= nz-2, cz].
 __global__ __launch_bounds__(kThreads) void dtwd_kernel(const float *__restrict__ x,
                                                         const float *__restrict__ wr, int nz, int ny,
-                                                        int nx, float scale,
+                                                        int nx, int cz, float scale,
                                                         const float *__restrict__ base,
                                                         float *__restrict__ out) {
     int64_t n = (int64_t)nz * ny * nx;
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(kThreads) void dtwd_kernel(const float *__restrict_
         double g0 = (double)wr[i - 1] * ((double)x[i] - (double)x[i - 1]);
         acc -= g1 - g0;
     }
-    if (nz > 2 && z >= 1 && z <= nz - 2) {
+    if (cz && z >= 1 && z <= nz - 2) {
         double g1 = (double)wr[i] * ((double)x[i + sz] - (double)x[i]);
         double g0 = (double)wr[i - sz] * ((double)x[i] - (double)x[i - sz]);
         acc -= g1 - g0;
@@ -213,36 +214,36 @@ int mct_sq_diff_f64(mct_ctx *ctx, const float *a, const float *b, const float *m
 }
 
 int mct_tv_weight(mct_ctx *ctx, const float *x, const float *prior, int32_t nz, int32_t ny,
-                  int32_t nx, double beta, float *wr) {
+                  int32_t nx, int32_t couple_z, double beta, float *wr) {
     MCT_CHECK(ctx && x && wr, "NULL argument");
     MCT_CHECK(nz > 0 && ny > 0 && nx > 0, "empty volume");
     int64_t n = (int64_t)nz * ny * nx;
     hipLaunchKernelGGL(tv_weight_kernel, dim3(mct::blocks_for(n, kThreads)), dim3(kThreads), 0,
-                       ctx->stream, x, prior, nz, ny, nx, beta, wr);
+                       ctx->stream, x, prior, nz, ny, nx, (int)(couple_z && nz > 2), beta, wr);
     MCT_LAUNCHED();
     return MCT_OK;
 }
 
 int mct_tv_value(mct_ctx *ctx, const float *x, const float *prior, int32_t nz, int32_t ny,
-                 int32_t nx, double beta, double *out) {
+                 int32_t nx, int32_t couple_z, double beta, double *out) {
     MCT_CHECK(ctx && x && out, "NULL argument");
     MCT_CHECK(nz > 0 && ny > 0 && nx > 0, "empty volume");
     int rc = mct::ensure_partials(ctx, kReduceBlocks);
     if (rc) return rc;
     hipLaunchKernelGGL(tv_value_kernel, dim3(kReduceBlocks), dim3(kThreads), 0, ctx->stream, x, prior,
-                       nz, ny, nx, beta, ctx->partials);
+                       nz, ny, nx, (int)(couple_z && nz > 2), beta, ctx->partials);
     MCT_LAUNCHED();
     return finish(ctx, out);
 }
 
 int mct_dtwd(mct_ctx *ctx, const float *x, const float *wr, int32_t nz, int32_t ny, int32_t nx,
-             float scale, const float *base, float *out) {
+             int32_t couple_z, float scale, const float *base, float *out) {
     MCT_CHECK(ctx && x && wr && out, "NULL argument");
     MCT_CHECK(nz > 0 && ny > 0 && nx > 0, "empty volume");
     MCT_CHECK(out != x && out != wr, "out must not alias x or wr");
     int64_t n = (int64_t)nz * ny * nx;
     hipLaunchKernelGGL(dtwd_kernel, dim3(mct::blocks_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       x, wr, nz, ny, nx, scale, base, out);
+                       x, wr, nz, ny, nx, (int)(couple_z && nz > 2), scale, base, out);
     MCT_LAUNCHED();
     return MCT_OK;
 }

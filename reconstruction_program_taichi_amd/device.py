@@ -117,19 +117,22 @@ class Context:
         call("mct_sq_diff_f64", self.h, _f32(a), _f32(b), _ptr(mb), a.numel(), _ptr(out))
         return out
 
-    def tv_weight(self, x, prior, beta: float, out):
+    def tv_weight(self, x, prior, beta: float, out, couple_z: bool = True):
         nz, ny, nx = x.shape
-        call("mct_tv_weight", self.h, _f32(x), _ptr(prior), nz, ny, nx, float(beta), _f32(out))
+        call("mct_tv_weight", self.h, _f32(x), _ptr(prior), nz, ny, nx, int(couple_z), float(beta),
+             _f32(out))
         return out
 
-    def tv_value(self, x, prior, beta: float, out):
+    def tv_value(self, x, prior, beta: float, out, couple_z: bool = True):
         nz, ny, nx = x.shape
-        call("mct_tv_value", self.h, _f32(x), _ptr(prior), nz, ny, nx, float(beta), _ptr(out))
+        call("mct_tv_value", self.h, _f32(x), _ptr(prior), nz, ny, nx, int(couple_z), float(beta),
+             _ptr(out))
         return out
 
-    def dtwd(self, x, wr, scale: float, base, out):
+    def dtwd(self, x, wr, scale: float, base, out, couple_z: bool = True):
         nz, ny, nx = x.shape
-        call("mct_dtwd", self.h, _f32(x), _f32(wr), nz, ny, nx, float(scale), _ptr(base), _f32(out))
+        call("mct_dtwd", self.h, _f32(x), _f32(wr), nz, ny, nx, int(couple_z), float(scale),
+             _ptr(base), _f32(out))
         return out
 
     def axpby(self, a: float, x, b: float, y, out):
