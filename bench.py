@@ -25,6 +25,11 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per back-projection launch of this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
+# (FETCH x2, WRITE x1); profiles/fdk_c1_kernels.md.  Keyed by
+# (storage, --bp-tile, --bp-batch); other settings report null.
+MEASURED_TRAFFIC = {("f32", 0, 0): 94.2e9}
 # Vector L1 fill rate of a CDNA CU: 64 B/clk; MI355X peak engine clock 2.4 GHz.
 L1_BYTES_PER_CLK_PER_CU = 64
 CLOCK_GHZ = 2.4
@@ -87,17 +92,41 @@ def main() -> int:
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     # bf16x3: split-BF16 Toeplitz GEMM on MFMA, 2.5e-6 rel-L2 vs FP64 (FP32-grade)
     ap.add_argument("--filter", default="bf16x3", choices=["f32", "bf16", "bf16x3"])
-    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--cpu-slices", type=int, default=32)
+    ap.add_argument("--serve-jobs", type=int, default=36,
+                    help="jobs in the served-stream comparison (0: skip it)")
+    ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bp-tile", type=int, default=0, help="slices per thread (0: library default)")
     ap.add_argument("--bp-batch", type=int, default=0, help="loads per batch (0: library default)")
     ap.add_argument("--sinogram", default="phantom", choices=["phantom", "random"],
                     help="analytic ellipsoid projections, or seeded uniform noise (profiler runs)")
+    ap.add_argument("--workload", default="fdk", choices=["fdk", "fbp2d", "fp3d", "ir"],
+                    help="fdk: the bench line; the others print their own measurement")
+    ap.add_argument("--fp-views", type=int, default=1080, help="fp3d: views projected")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only copy a buffer of this many MiB 5 times (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import torch
+
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {args.copy_calibration_mib} MiB x 5")
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the reconstruction path runs on a single device)", file=sys.stderr)
         return 2
+    if args.workload != "fdk":
+        from reconstruction_program_taichi_amd import workloads
+
+        print(json.dumps(workloads.run(args.workload, args)), flush=True)
+        return 0
 
     import torch
 
@@ -166,7 +195,8 @@ def main() -> int:
                    "sinogram_storage": args.storage, "ramp_filter": args.filter,
                    "bp_ms": round(bp_ms, 3), "bp_gups": round(U / (bp_ms * 1e-3) / 1e9, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((args.storage, args.bp_tile, args.bp_batch)),
                      "kernel": "bp_cone_axial_kernel",
                      "model": f"gather bytes: 4 taps x {tap_bytes} B per update + 4 B per voxel; "
                               "the taps are served by L1/L2, so frac > 1 is expected",
@@ -182,8 +212,46 @@ def main() -> int:
             denom = float(np.linalg.norm(check)) or 1.0
             cb["rel_l2_vs_gpu"] = float(np.linalg.norm(gpu_slab - check) / denom)
         line["cpu_baseline"] = cb
+    bp_times.clear()
+    plan.backproject_fbp = orig_bp
+    fbp.last_filtered = None
+    del work, vol, raw
+    fbp.plan().close()
+    fbp.ctx().close()
+    torch.cuda.synchronize()
+    if args.serve_jobs > 0:
+        torch.cuda.empty_cache()
+        line["served"] = served_stream(args.serve_jobs, args.serve_qps)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def served_stream(n_jobs: int, qps: float) -> dict:
+    """Mixed FBP / FDK / CGLS-TV stream (Poisson arrivals, seed 3, FP16
+    sinogram storage): continuous-batching server vs one job at a time."""
+    from reconstruction_program_taichi_amd import serve
+
+    jobs = serve.make_mix(n_jobs, qps, seed=3, storage="f16")
+    plain = [j.fresh() for j in jobs]
+    warm = [j.fresh() for j in jobs[:6]]
+    srv = serve.Server(hbm_budget=96 << 30)
+    try:
+        srv.run(warm)  # compile / allocate once
+        st = srv.run(jobs)
+    finally:
+        srv.close()
+    sp = serve.run_plain(plain)
+    same = all(a.result is not None and b.result is not None and bool((a.result == b.result).all())
+               for a, b in zip(jobs, plain))
+    mix = {}
+    for j in jobs:
+        mix[j.kind] = mix.get(j.kind, 0) + 1
+    for j in jobs + plain + warm:
+        j.result = None
+        j.sgm_host = None
+    return {"qps_offered": qps, "mix": mix, "served": st, "plain": sp,
+            "throughput_ratio": round(st["jobs_per_s"] / sp["jobs_per_s"], 3),
+            "results_identical": same}
 
 
 if __name__ == "__main__":

@@ -117,7 +117,9 @@ int mct_abi_version(void);
  * *g and the per-view tables derived from them (cos/sin of the view angle with
  * and without ImageRotation, the angular interval of run_mgfbp.py:1105-1111).
  * cos/sin are evaluated in double precision and rounded to FP32, so every
- * kernel sees the same table.  The plan keeps no pointer into *g. */
+ * kernel sees the same table.  The plan keeps no pointer into *g.  Operations
+ * on a plan run on its context's stream; mct_plan_destroy does not use the
+ * context, so plans and contexts may be destroyed in either order. */
 int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out);
 int mct_plan_destroy(mct_plan *plan);
 
@@ -163,6 +165,26 @@ int64_t mct_packed_bytes(int32_t rows, int32_t views, int32_t n_u, int32_t dtype
  * vol: DEVICE FP32 [img_dim_z][img_dim][img_dim], overwritten. */
 int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
                         float *vol);
+
+/* One job of a batched back-projection: a flat-detector axial plan, its
+ * packed filtered sinogram (QUAD for cone beam, PAIR for fan beam) and the
+ * output volume. */
+typedef struct mct_bp_job {
+    mct_plan *plan;
+    const void *filt;
+    int32_t dtype;
+    int32_t layout;
+    float *vol;
+} mct_bp_job;
+
+/* Mgfbp.BackProjectionPixelDriven run_mgfbp.py:1062-1187 for njobs volumes or
+ * slice stacks of different geometry and size in ONE persistent launch (the
+ * reference back-projects one file per kernel call, run_mgfbp.py:58-81).
+ * Each job's result equals mct_backproject_fbp's for that job alone.  All
+ * jobs share one dtype; plans must come from ctx.  The job table is copied
+ * to the device through a context-owned pinned buffer, so calls on one
+ * context are serialised on the host by that copy. */
+int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs);
 
 /* Tile shape of the axial back-projector (autotuner hook): zc slices per
  * thread (4, 8, 16 or 32) and, for cone beam, batch taps loads issued

@@ -53,6 +53,13 @@ class Geometry(C.Structure):
     ]
 
 
+class BpJob(C.Structure):
+    """Mirror of struct mct_bp_job (include/mct.h)."""
+
+    _fields_ = [("plan", C.c_void_p), ("filt", C.c_void_p), ("dtype", C.c_int32),
+                ("layout", C.c_int32), ("vol", C.c_void_p)]
+
+
 _P = C.c_void_p
 _I32, _I64, _F32, _F64, _U64 = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint64
 
@@ -73,6 +80,7 @@ SIGNATURES = {
     "mct_packed_bytes": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "mct_backproject_fbp": (C.c_int, [_P, _P, _I32, _I32, _P]),
     "mct_set_bp_tile": (C.c_int, [_P, _I32, _I32]),
+    "mct_backproject_batch": (C.c_int, [_P, C.POINTER(BpJob), _I32]),
     "mct_forward_project": (C.c_int, [_P, _P, _P, _P, _I32]),
     "mct_ray_mask": (C.c_int, [_P, _P]),
     "mct_add_noise": (C.c_int, [_P, _P, _I64, _F32, _U64]),

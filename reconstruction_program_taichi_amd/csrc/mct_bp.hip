@@ -140,9 +140,7 @@ __device__ __forceinline__ float bilerp(float4 q, float ru, float rv) {
 
 // Logical tile of this workgroup: 16 x 16 columns (four 8 x 8 wavefront
 // tiles) and one chunk of ZC slices.
-__device__ __forceinline__ void decode_tile(const BpArgs &a, int &ix, int &iy, int &zc_idx) {
-    int nb = a.tiles_x * a.tiles_y * a.nzc;
-    int b = mct::xcd_remap((int)blockIdx.x, nb, a.nxcd);
+__device__ __forceinline__ void decode_tile(const BpArgs &a, int b, int &ix, int &iy, int &zc_idx) {
     int t = b % (a.tiles_x * a.tiles_y);
     zc_idx = b / (a.tiles_x * a.tiles_y);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -231,10 +229,10 @@ __device__ __forceinline__ void v_interval(const BpArgs &a, float mag, int z0, i
 // Cone beam, axial view, QUAD taps; ZC slices per thread, loads in batches
 // of B.
 template <int ZC, int B, int DT>
-__global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
+__device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
     static_assert(ZC % B == 0, "batch must divide the slice chunk");
     int ix, iy, zci;
-    decode_tile(a, ix, iy, zci);
+    decode_tile(a, block, ix, iy, zci);
     if (ix >= a.nx || iy >= a.nx) return;
     const int z0 = zci * ZC;
     const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
@@ -293,11 +291,16 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
         if (k < nk) out[(size_t)(z0 + k) * plane] = (k >= lo && k <= hi) ? acc[k] : 0.f;
 }
 
+template <int ZC, int B, int DT>
+__global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
+    cone_tile<ZC, B, DT>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd));
+}
+
 // Fan beam, axial view, PAIR taps; slice z reads detector row z.
 template <int ZC, int DT>
-__global__ __launch_bounds__(256) void bp_fan_axial_kernel(BpArgs a) {
+__device__ __forceinline__ void fan_tile(const BpArgs &a, int block) {
     int ix, iy, zci;
-    decode_tile(a, ix, iy, zci);
+    decode_tile(a, block, ix, iy, zci);
     if (ix >= a.nx || iy >= a.nx) return;
     const int z0 = zci * ZC;
     const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
@@ -330,6 +333,39 @@ __global__ __launch_bounds__(256) void bp_fan_axial_kernel(BpArgs a) {
 #pragma unroll
     for (int k = 0; k < ZC; ++k)
         if (k < nk) out[(size_t)(z0 + k) * plane] = alive ? acc[k] : 0.f;
+}
+
+template <int ZC, int DT>
+__global__ __launch_bounds__(256) void bp_fan_axial_kernel(BpArgs a) {
+    fan_tile<ZC, DT>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd));
+}
+
+// Back-projection of many volumes / slice stacks of different geometry in one
+// launch.  The work items (one 16 x 16-column x ZC-slice tile of one job)
+// are numbered job after job; offsets[j] is the first item of job j
+// (offsets[njobs] = total).  A fixed grid walks the items in a static round
+// robin, so the launch is deterministic and needs no counters (it can be
+// captured in a graph and replayed).
+template <int DT>
+__global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict__ jobs,
+                                                       const int *__restrict__ offsets, int njobs,
+                                                       int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = njobs - 1;  // last job with offsets[job] <= item
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (offsets[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BpArgs a = jobs[lo];
+        const int local = item - offsets[lo];
+        if (a.cone)
+            cone_tile<16, 16, DT>(a, local);
+        else
+            fan_tile<8, DT>(a, local);
+    }
 }
 
 // Voxel position for any ReconViewMode, run_mgfbp.py:1086-1104.
@@ -449,7 +485,7 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
 template <int ZC>
 __global__ __launch_bounds__(256) void bp_ir_axial_kernel(BpArgs a) {
     int ix, iy, zci;
-    decode_tile(a, ix, iy, zci);
+    decode_tile(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd), ix, iy, zci);
     if (ix >= a.nx || iy >= a.nx) return;
     const int z0 = zci * ZC;
     const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
@@ -677,6 +713,77 @@ int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, 
     a.nzc = (g.img_dim_z + zc - 1) / zc;
     int nblocks = a.tiles_x * a.tiles_y * a.nzc;
     hipLaunchKernelGGL(bp_ir_axial_kernel<8>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
+int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
+    MCT_CHECK(ctx && jobs, "NULL argument");
+    MCT_CHECK(njobs >= 0 && njobs <= 4096, "njobs must be in [0, 4096] (got %d)", njobs);
+    if (njobs == 0) return MCT_OK;
+    const int dtype = jobs[0].dtype;
+    MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
+    const size_t args_bytes = sizeof(BpArgs) * (size_t)njobs;
+    const size_t bytes = args_bytes + sizeof(int) * ((size_t)njobs + 1);
+    if (ctx->batch_cap < (int64_t)bytes) {
+        if (ctx->batch_host) {
+            MCT_HIP(hipStreamSynchronize(ctx->stream));
+            hipHostFree(ctx->batch_host);
+            hipFree(ctx->batch_dev);
+            ctx->batch_host = ctx->batch_dev = nullptr;
+            ctx->batch_cap = 0;
+        }
+        size_t cap = bytes < 65536 ? 65536 : bytes;
+        MCT_HIP(hipHostMalloc(&ctx->batch_host, cap, hipHostMallocDefault));
+        MCT_HIP(hipMalloc(&ctx->batch_dev, cap));
+        ctx->batch_cap = (int64_t)cap;
+    }
+    if (!ctx->batch_ev) MCT_HIP(hipEventCreateWithFlags(&ctx->batch_ev, hipEventDisableTiming));
+    // the previous batch's table copy must have left the staging buffer
+    MCT_HIP(hipEventSynchronize(ctx->batch_ev));
+    BpArgs *args = reinterpret_cast<BpArgs *>(ctx->batch_host);
+    int *offs = reinterpret_cast<int *>(reinterpret_cast<char *>(ctx->batch_host) + args_bytes);
+    int64_t total = 0;
+    for (int i = 0; i < njobs; ++i) {
+        const mct_bp_job &jb = jobs[i];
+        MCT_CHECK(jb.plan && jb.filt && jb.vol, "job %d: NULL plan, sinogram or volume", i);
+        MCT_CHECK(jb.plan->ctx == ctx, "job %d: plan belongs to another context", i);
+        MCT_CHECK(jb.dtype == dtype, "job %d: all jobs of a batch must share one dtype", i);
+        const mct_geometry &g = jb.plan->g;
+        MCT_CHECK(g.view_mode == 1 && !g.apply_pmatrix && !g.curved,
+                  "job %d: batched back-projection supports flat-detector axial jobs", i);
+        MCT_CHECK(jb.layout == (g.cone ? MCT_QUAD : MCT_PAIR), "job %d: cone reads QUAD, fan PAIR", i);
+        if (!g.cone) MCT_CHECK(g.n_v == g.img_dim_z, "job %d: fan beam needs one row per slice", i);
+        if (g.cone) MCT_CHECK(g.det_h > 0.f, "job %d: cone beam needs det_h > 0", i);
+        BpArgs a = make_args(jb.plan);
+        a.sino = jb.filt;
+        a.layout = jb.layout;
+        a.vol = jb.vol;
+        const int zc = g.cone ? 16 : 8;
+        a.tiles_x = (g.img_dim + 15) / 16;
+        a.tiles_y = a.tiles_x;
+        a.nzc = (g.img_dim_z + zc - 1) / zc;
+        args[i] = a;
+        offs[i] = (int)total;
+        total += (int64_t)a.tiles_x * a.tiles_y * a.nzc;
+        MCT_CHECK(total < (int64_t)1 << 30, "batch too large");
+    }
+    offs[njobs] = (int)total;
+    MCT_HIP(hipMemcpyAsync(ctx->batch_dev, ctx->batch_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MCT_HIP(hipEventRecord(ctx->batch_ev, ctx->stream));
+    const BpArgs *dargs = reinterpret_cast<const BpArgs *>(ctx->batch_dev);
+    const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(ctx->batch_dev) + args_bytes);
+    int64_t grid = (int64_t)ctx->cu_count * 8;
+    grid = grid < total ? grid : total;
+    if (dtype == MCT_F32)
+        hipLaunchKernelGGL(bp_batch_kernel<MCT_F32>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, dargs,
+                           doffs, njobs, (int)total);
+    else if (dtype == MCT_F16)
+        hipLaunchKernelGGL(bp_batch_kernel<MCT_F16>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, dargs,
+                           doffs, njobs, (int)total);
+    else
+        hipLaunchKernelGGL(bp_batch_kernel<MCT_BF16>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, dargs,
+                           doffs, njobs, (int)total);
     MCT_LAUNCHED();
     return MCT_OK;
 }

@@ -171,6 +171,9 @@ int mct_destroy(mct_ctx *ctx) {
     if (ctx->toep) hipFree(ctx->toep);
     if (ctx->taps_dev) hipFree(ctx->taps_dev);
     if (ctx->vkern_dev) hipFree(ctx->vkern_dev);
+    if (ctx->batch_host) hipHostFree(ctx->batch_host);
+    if (ctx->batch_dev) hipFree(ctx->batch_dev);
+    if (ctx->batch_ev) hipEventDestroy(ctx->batch_ev);
     delete ctx;
     return MCT_OK;
 }
@@ -220,6 +223,7 @@ int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
 
     mct_plan *p = new mct_plan();
     p->ctx = ctx;
+    p->device = ctx->device;
     p->g = *g;
     p->g.angles = p->g.u = p->g.v = p->g.pmatrix = p->g.center_z = p->g.matrix_a = p->g.x_s = nullptr;
     const int nv = g->n_views;
@@ -294,8 +298,10 @@ int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
 
 int mct_plan_destroy(mct_plan *p) {
     if (!p) return MCT_OK;
-    hipSetDevice(p->ctx->device);
-    hipStreamSynchronize(p->ctx->stream);
+    // The plan's context may already be destroyed (callers may release the
+    // two in either order), so nothing here reads p->ctx; hipFree waits for
+    // the device's outstanding work before it releases the memory.
+    hipSetDevice(p->device);
     free_plan_arrays(p);
     delete p;
     return MCT_OK;

@@ -27,10 +27,16 @@ struct mct_ctx {
     int64_t taps_cap = 0;
     float *vkern_dev = nullptr;  // device copy of the vertical Gaussian taps
     int64_t vkern_cap = 0;
+    // job table of mct_backproject_batch: pinned staging, device copy, and the
+    // event that marks the staging buffer free again
+    void *batch_host = nullptr, *batch_dev = nullptr;
+    int64_t batch_cap = 0;
+    hipEvent_t batch_ev = nullptr;
 };
 
 struct mct_plan {
-    mct_ctx *ctx = nullptr;
+    mct_ctx *ctx = nullptr;  // used by the operations, never by mct_plan_destroy
+    int device = 0;
     mct_geometry g{};  // scalars; the host pointers are cleared
     // device copies of the geometry arrays
     float *angles = nullptr, *u = nullptr, *v = nullptr;

@@ -105,6 +105,16 @@ class Context:
              LAYOUTS[layout])
         return out
 
+    def backproject_batch(self, jobs) -> None:
+        """jobs: iterable of (plan, packed filtered sinogram, dtype, layout, vol)."""
+        jobs = list(jobs)
+        arr = (_abi.BpJob * max(len(jobs), 1))()
+        for i, (plan, filt, dtype, layout, vol) in enumerate(jobs):
+            if plan.ctx is not self:
+                raise ValueError("plan belongs to another context")
+            arr[i] = _abi.BpJob(plan.h.value, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
+        call("mct_backproject_batch", self.h, arr, len(jobs))
+
     def add_noise(self, sgm: torch.Tensor, photons: float, seed: int) -> torch.Tensor:
         call("mct_add_noise", self.h, _f32(sgm), sgm.numel(), float(photons), int(seed) & (2**64 - 1))
         return sgm

@@ -1,0 +1,134 @@
+"""Timed synthetic workloads besides the cone FDK bench line.
+
+  fbp2d: fan FBP 512 x 512 at 0.3 mm, 720 views over 360 deg, 800 elements at
+         0.4 mm, SID 500 / SDD 1000 mm, Ram-Lak; 10 seeded random ellipses.
+  fp3d:  cone forward projection of a 512 x 512 x 256 volume (0.3 mm voxels,
+         30 seeded random ellipsoids) into 1080 views x 256 rows x 1024
+         columns at 0.4 mm, SID 750 / SDD 1250 mm, step 0.2 of the voxel
+         diagonal (optionally a leading subset of the views).
+  ir:    IRN-CGLS-TV on 64 fan slices of 512 x 512 (geometry of fbp2d with
+         120 views), 5 reweighting passes x 20 CG iterations, Lambda 1e-3,
+         BetaTV 1e-6, slices coupled through TV as the reference does.
+Each returns a dict (printed as one JSON line by bench.py).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import replace
+
+import numpy as np
+import torch
+
+from . import phantom
+from .device import Plan
+
+
+def _fan_cfg(views: int, slices: int = 1) -> dict:
+    return {"InputDir": "/nonexistent", "OutputDir": "/tmp/mct_workloads", "InputFiles": "x",
+            "OutputFilePrefix": "", "OutputFileReplace": ["a", "b"],
+            "DetectorElementCountHorizontal": 800, "DetectorElementWidth": 0.4,
+            "DetectorOffsetHorizontal": 0.0, "DetectorElementCountVertical": slices, "Views": views,
+            "TotalScanAngle": 360.0, "SourceIsocenterDistance": 500.0,
+            "SourceDetectorDistance": 1000.0, "ImageDimension": 512, "PixelSize": 0.3,
+            "HammingFilter": 1.0}
+
+
+def _sync_time(fn, reps: int) -> float:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def fbp2d(args) -> dict:
+    from .fbp import Mgfbp
+
+    dev = torch.device("cuda", 0)
+    fbp = Mgfbp(_fan_cfg(720))
+    spec = fbp.geometry_spec()
+    r_fov = min(256 * 0.3, 400 * 0.4 * 500 / 1000) - 2 * 0.3
+    ell = phantom.random_ellipsoids(10, seed=0, r_fov=r_fov)
+    raw = phantom.project(ell, spec.angles, spec.u, spec.v, spec.sid, spec.sdd, device=dev)
+    work = torch.empty_like(raw)
+    vol = torch.empty((1, 512, 512), dtype=torch.float32, device=dev)
+
+    def step():
+        work.copy_(raw)
+        fbp.reconstruct(work, vol)
+
+    for _ in range(3):
+        step()
+    t = _sync_time(step, max(args.steps, 20))
+    U = 512 * 512 * 720
+    return {"workload": "fan FBP 512x512, 720 views, 800 elements (FP32, Ram-Lak)",
+            "ms_per_step": round(t * 1e3, 4), "gups": round(U / t / 1e9, 2),
+            "ramp_filter": fbp.filter_method}
+
+
+def fp3d(args) -> dict:
+    from .fpj import Mgfpj
+
+    dev = torch.device("cuda", 0)
+    nv = int(args.fp_views)
+    cfg = {"InputDir": "/nonexistent", "OutputDir": "/tmp/mct_workloads", "InputFiles": "x",
+           "OutputFileReplace": ["a", "b"], "ImageDimension": 512, "PixelSize": 0.3,
+           "ImageDimensionZ": 256, "ConeBeam": True, "VoxelHeight": 0.3,
+           "DetectorElementCountHorizontal": 1024, "DetectorElementWidth": 0.4,
+           "DetectorElementCountVertical": 256, "DetectorElementHeight": 0.4, "Views": 1080,
+           "SourceIsocenterDistance": 750.0, "SourceDetectorDistance": 1250.0}
+    fpj = Mgfpj(cfg)
+    spec = fpj.geometry_spec()
+    if nv < spec.n_views:  # leading subset of the views (same per-view work)
+        spec = replace(spec, n_views=nv, angles=spec.angles[:nv])
+    ell = phantom.random_ellipsoids(30, seed=2, r_fov=70.0, z_half=30.0)
+    vol = phantom.voxelize(ell, 512, 256, 0.3, 0.3, clip_r=76.0, supersample=1, device=dev)
+    out = torch.empty((256, nv, 1024), dtype=torch.float32, device=dev)
+    plan = Plan(fpj.ctx(), spec)
+    plan.forward_project(vol, out)  # warm-up
+    t = _sync_time(lambda: plan.forward_project(vol, out), max(1, min(args.steps, 3)))
+    diag = float(np.sqrt(np.float32(2) * np.float32(0.3) ** 2 + np.float32(0.3) ** 2))
+    dim = np.float32(512 * 0.3)
+    D = float(np.sqrt(np.float32(2) * dim * dim + np.float32(256 * 0.3) ** 2))
+    steps = int(np.floor(np.float32(D) / (np.float32(0.2) * np.float32(diag))))
+    rays = 256 * nv * 1024
+    S = rays * steps
+    return {"workload": f"cone forward projection 512x512x256 -> {nv} views x 256 x 1024",
+            "ms": round(t * 1e3, 3), "rays": rays, "steps_per_ray": steps,
+            "samples_per_s": round(S / t / 1e9, 2), "unit": "G samples/s (reference step count)"}
+
+
+def ir(args) -> dict:
+    from .ir import Mgfbp_ir
+
+    dev = torch.device("cuda", 0)
+    cfg = {**_fan_cfg(120, 64), "Lambda": 1e-3, "BetaTV": 1e-6, "NumberOfIRNIterations": 5,
+           "NumberOfIterations": 20}
+    irr = Mgfbp_ir(cfg)
+    spec = irr.ir_geometry_spec()
+    r_fov = min(256 * 0.3, 400 * 0.4 * 500 / 1000) - 2 * 0.3
+    sg = torch.empty((64, 120, 800), dtype=torch.float32, device=dev)
+    for s in range(64):  # an independent random phantom per slice
+        ell = phantom.random_ellipsoids(10, seed=100 + s, r_fov=r_fov)
+        sg[s:s + 1] = phantom.project(ell, spec.angles, spec.u, spec.v[:1], spec.sid, spec.sdd, device=dev)
+    x0 = torch.zeros((64, 512, 512), dtype=torch.float32, device=dev)
+    n_irn, n_it = irr.num_irn_iter, irr.num_iter
+    irr.num_irn_iter, irr.num_iter = 0, 1
+    irr.solve(sg, x0)  # warm-up (allocation, plan)
+    irr.num_irn_iter, irr.num_iter = n_irn, n_it
+    irr.num_iter_runned = 0
+    irr.loss = np.empty(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    irr.solve(sg, x0)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    n_cg = (1 + irr.num_irn_iter) * irr.num_iter
+    return {"workload": "IRN-CGLS-TV, 64 fan slices 512x512, 120 views, (1+5) passes x 20 CG iterations",
+            "total_s": round(t, 3), "cg_iterations": n_cg, "ms_per_cg_iteration": round(t / n_cg * 1e3, 3),
+            "fp_bp_pairs": irr.num_iter_runned, "loss_first_last": [float(irr.loss[0]), float(irr.loss[-1])]}
+
+
+def run(name: str, args) -> dict:
+    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir}[name](args)

@@ -91,6 +91,17 @@ def test_filter_rows_mfma(ctx, method, tol, n, rows):
     assert err <= tol
 
 
+def test_context_destroyed_before_plan():
+    """Garbage collection may finalise a context before its plans: destroying
+    a plan must not read its (already freed) context."""
+    from reconstruction_program_taichi_amd.device import Context
+
+    c = Context(0)
+    p = Plan(c, fan_case(nx=16, nv=8, nu=20))
+    c.close()
+    p.close()
+
+
 def test_filter_rows_empty(ctx):
     out = torch.empty((0, 16), dtype=torch.float32, device=DEV)
     inp = torch.empty((0, 16), dtype=torch.float32, device=DEV)

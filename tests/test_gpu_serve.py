@@ -1,0 +1,101 @@
+"""GPU tests of batched back-projection and the serving loop.
+
+Tolerance: exact.  The batched launch runs the single-job tile code and the
+CG reductions are deterministic, so a job's result must be bit-identical
+alone and in a batch / in the served mix.
+"""
+import numpy as np
+import pytest
+import torch
+
+from reconstruction_program_taichi_amd import serve
+from reconstruction_program_taichi_amd.device import Plan
+from tests._cases import cone_case, fan_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _single(ctx, spec, f, dtype):
+    layout = "quad" if spec.cone else "pair"
+    plan = Plan(ctx, spec)
+    packed = ctx.pack(f, dtype, layout)
+    vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
+    plan.backproject_fbp(packed, dtype, layout, vol)
+    return plan, packed, layout, vol
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_batched_bp_equals_single_jobs(ctx, dtype):
+    specs = [fan_case(nx=48, nv=60, nu=80, rows=3), cone_case(nx=40, nz=21, nv=50),
+             fan_case(nx=33, nv=45, nu=64, rows=1, scan_deg=200.0), cone_case(nx=56, nz=37, nv=64, top_row=True)]
+    rng = np.random.default_rng(0)
+    singles, jobs = [], []
+    for s in specs:
+        f = torch.from_numpy(rng.standard_normal((s.n_v, s.n_views, s.n_u)).astype(np.float32)).cuda()
+        plan, packed, layout, vol = _single(ctx, s, f, dtype)
+        singles.append(vol.clone())
+        out = torch.full_like(vol, 7.0)
+        jobs.append((plan, packed, dtype, layout, out))
+    ctx.backproject_batch(jobs)
+    torch.cuda.synchronize()
+    for want, job in zip(singles, jobs):
+        assert torch.equal(job[4], want)
+
+
+def test_batched_bp_rejects_mixed_dtypes(ctx):
+    from reconstruction_program_taichi_amd._abi import MctError
+
+    s = fan_case(nx=16, nv=20, nu=24, rows=1)
+    f = torch.rand((1, 20, 24), device="cuda")
+    plan, p32, layout, vol = _single(ctx, s, f, "f32")
+    p16 = ctx.pack(f, "f16", layout)
+    with pytest.raises(MctError):
+        ctx.backproject_batch([(plan, p32, "f32", layout, vol), (plan, p16, "f16", layout, vol)])
+
+
+def _small_mix(n=14, seed=11):
+    return serve.make_mix(n, qps=400.0, seed=seed, scale=0.12, prototypes_per_kind=1)
+
+
+def test_served_results_equal_plain_results():
+    jobs = _small_mix()
+    plain = [j.fresh() for j in jobs]
+    srv = serve.Server(hbm_budget=8 << 30)
+    try:
+        st = srv.run(jobs)
+    finally:
+        srv.close()
+    sp = serve.run_plain(plain)
+    assert st["jobs"] == len(jobs) and st["rejected"] == 0
+    kinds = set()
+    for a, b in zip(jobs, plain):
+        kinds.add(a.kind)
+        assert a.result is not None and b.result is not None
+        assert torch.equal(a.result, b.result), f"job {a.jid} ({a.kind}) differs"
+        assert bool(torch.isfinite(a.result).all())
+    assert kinds == {"fbp", "fdk", "cgls"}
+    print(f"served {st}\nplain {sp}")
+
+
+def test_admission_waits_for_budget_and_rejects_oversized():
+    jobs = _small_mix(n=8, seed=12)
+    big = max(j.nbytes() for j in jobs)
+    # room for the largest job only: jobs run one after another, none rejected
+    srv = serve.Server(hbm_budget=big)
+    try:
+        st = srv.run(jobs)
+    finally:
+        srv.close()
+    assert st["rejected"] == 0 and all(j.result is not None for j in jobs)
+    # a budget below the largest job rejects exactly the jobs that cannot fit
+    jobs2 = _small_mix(n=8, seed=12)
+    limit = sorted(j.nbytes() for j in jobs2)[len(jobs2) // 2]
+    srv = serve.Server(hbm_budget=limit)
+    try:
+        st = srv.run(jobs2)
+    finally:
+        srv.close()
+    too_big = [j for j in jobs2 if j.nbytes() > limit]
+    assert st["rejected"] == len(too_big) > 0
+    assert all(j.result is None for j in too_big)
+    assert all(j.result is not None for j in jobs2 if j.nbytes() <= limit)
