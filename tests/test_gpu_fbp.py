@@ -91,6 +91,22 @@ def test_filter_rows_mfma(ctx, method, tol, n, rows):
     assert err <= tol
 
 
+@pytest.mark.parametrize("rows", [1, 7, 48])
+def test_filter_vertical_matches_oracle(ctx, rows):
+    from reconstruction_program_taichi_amd import geometry as geo
+
+    rng = np.random.default_rng(rows)
+    x = rng.standard_normal((rows, 9, 33)).astype(np.float32)
+    kern = geo.gaussian_taps(rows, 0.6, 1.2)
+    out = torch.empty_like(dev(x))
+    ctx.filter_vertical(dev(x), out, kern, 0.6)
+    ref32 = recon.filter_vertical(x, kern, 0.6, np.float32)
+    ref64 = recon.filter_vertical(x.astype(np.float64), kern.astype(np.float64), 0.6, np.float64)
+    got = host(out)
+    assert rel_l2(got, ref32) <= 1e-6
+    assert rel_l2(got, ref64) <= 1e-5
+
+
 def test_context_destroyed_before_plan():
     """Garbage collection may finalise a context before its plans: destroying
     a plan must not read its (already freed) context."""

@@ -189,6 +189,56 @@ def test_fbp_of_disc_reproduces_disc():
     assert np.abs(outer).mean() / mu < 0.03
 
 
+@pytest.mark.parametrize("rows", [1, 5, 40])
+def test_filter_vertical_is_a_truncated_convolution(rows):
+    """The vertical Gaussian pass equals np.convolve along the row axis with
+    the taps outside |offset| <= 10 zeroed, times the row height; a Gaussian
+    normalised to 1/h leaves a constant column constant on rows at least 10
+    away from both ends."""
+    h = 0.7
+    kern = geo.gaussian_taps(rows, h, 1.5).astype(np.float64)
+    rng = np.random.default_rng(rows)
+    x = rng.standard_normal((rows, 3, 4))
+    kt = kern.copy()
+    off = np.arange(2 * rows - 1) - (rows - 1)
+    kt[np.abs(off) > 10] = 0
+    want = np.apply_along_axis(lambda c: np.convolve(c, kt)[rows - 1:2 * rows - 1], 0, x) * h
+    assert np.allclose(recon.filter_vertical(x, kern, h), want, atol=1e-12)
+    const = recon.filter_vertical(np.ones((rows, 1, 1)), kern, h)[:, 0, 0]
+    interior = const[10:rows - 10]
+    assert np.allclose(interior, 1.0, atol=1e-6)
+
+
+def test_fbp_of_offcentre_disc_reproduces_it_in_place():
+    """Analytic fan-beam line integrals of an off-centre disc (distance of each
+    source-to-element line from the disc centre), FBP'd by the oracle: the
+    disc comes back at its position, not mirrored or rotated."""
+    nx, nviews, nu = 64, 360, 256
+    R, mu, pix, w, sid, sdd = 5.0, 0.02, 0.5, 0.25, 200.0, 400.0
+    cxy = np.array([6.0, -4.0])  # object coordinates, mm
+    spec = fan_case(nx=nx, nv=nviews, nu=nu, pix=pix, w=w, sid=sid, sdd=sdd, rows=1)
+    u = spec.u.astype(np.float64)
+    sgm = np.zeros((1, nviews, nu))
+    for j, b in enumerate(spec.angles.astype(np.float64)):
+        cb, sb = np.cos(b), np.sin(b)
+        S = sid * np.array([cb, sb])
+        D = np.stack([(sid - sdd) * cb - u * sb, (sid - sdd) * sb + u * cb], axis=1)
+        e = D - S
+        d = np.abs(e[:, 0] * (cxy[1] - S[1]) - e[:, 1] * (cxy[0] - S[0])) / np.hypot(e[:, 0], e[:, 1])
+        sgm[0, j] = 2 * np.sqrt(np.clip(R * R - d * d, 0, None)) * mu
+    ws = recon.weight_sinogram(sgm, spec.u, spec.v, spec.angles, sdd, spec.total_scan_angle, False,
+                               dtype=np.float64)
+    fl = recon.filter_rows(ws, recon.ramlak_taps(nu, w), w, np.float64)
+    img = recon.backproject_fbp(fl, oracle_dict(spec), np.float64)[0]
+    xs, ys = _image_xy(nx, pix)
+    X, Y = np.meshgrid(xs, ys)
+    r = np.hypot(X - cxy[0], Y - cxy[1])
+    assert abs(img[r < 0.6 * R].mean() / mu - 1) < 0.03
+    for mirror in ([-1, 1], [1, -1], [-1, -1]):
+        rm = np.hypot(X - mirror[0] * cxy[0], Y - mirror[1] * cxy[1])
+        assert np.abs(img[rm < 0.6 * R]).mean() / mu < 0.05
+
+
 # ---------------------------------------------------- forward projection
 def test_fp_of_uniform_disc_equals_chord_length():
     """FP (FP64 oracle) of a slice holding a centred uniform disc: chord
@@ -232,6 +282,128 @@ def test_fp_is_linear():
     fb = recon.forward_project(b, g, dtype=np.float64)
     fab = recon.forward_project(2 * a - 3 * b, g, dtype=np.float64)
     assert np.allclose(fab, 2 * fa - 3 * fb, atol=1e-10)
+
+
+def _image_xy(nx, pix):
+    """Object coordinates of pixel centres: x grows with the column index,
+    y falls with the row index (run_mgfbp.py:1086-1104)."""
+    h = (nx - 1) / 2.0
+    i = np.arange(nx)
+    return pix * (i - h), -pix * (i - h)
+
+
+def _project_point(x, y, beta, sid, sdd):
+    """Flat-detector coordinate of the ray through object point (x, y) for a
+    gantry at angle beta: source at sid (cos b, sin b), detector line at
+    distance sdd; the point seen from the rotated frame is R(-b) (x, y)."""
+    xg = x * np.cos(beta) + y * np.sin(beta)
+    yg = -x * np.sin(beta) + y * np.cos(beta)
+    return sdd * yg / (sid - xg)
+
+
+def test_fp_of_offcentre_blob_peaks_at_its_projection():
+    """The FP oracle's sinogram of an off-centre blob is centred, in every
+    view, on the detector coordinate where the blob's centre projects: pins
+    the rotation direction, the detector orientation and the image axes."""
+    nx, pix = 64, 0.5
+    spec = fan_case(nx=nx, nv=24, nu=200, pix=pix, w=0.4, sid=200.0, sdd=400.0, rows=1)
+    g = oracle_dict(spec)
+    ix, iy, s = 47, 18, 1.5
+    yy, xx = np.mgrid[0:nx, 0:nx]
+    img = np.exp(-((xx - ix) ** 2 + (yy - iy) ** 2) / (2 * s * s))
+    sgm = recon.forward_project(img[None], g, dtype=np.float64)[0]
+    xs, ys = _image_xy(nx, pix)
+    u = spec.u.astype(np.float64)
+    for j, beta in enumerate(spec.angles.astype(np.float64)):
+        want = _project_point(xs[ix], ys[iy], beta, 200.0, 400.0)
+        got = float((sgm[j] * u).sum() / sgm[j].sum())
+        assert abs(got - want) < 0.25 * 0.4, (j, got, want)
+
+
+def test_bp_ir_of_a_ray_pair_is_the_band_between_the_rays():
+    """Unfiltered IR back-projection of a sinogram holding ones at detector
+    elements k, k+1 of one view: every pixel strictly between the two rays
+    (lines from the source through the element centres) receives the chord
+    length of its source ray through a pixel column, pix / max(|cos|, |sin|);
+    pixels beyond the neighbouring rays receive nothing; pixels that leave the
+    detector in some view are zero (full-scan FOV disc)."""
+    nx, pix, sid, sdd, w = 64, 0.5, 200.0, 400.0, 0.5
+    spec = fan_case(nx=nx, nv=8, nu=120, pix=pix, w=w, sid=sid, sdd=sdd, rows=1)
+    g = oracle_dict(spec)
+    j, k = 3, 52
+    sgm = np.zeros((1, 8, 120))
+    sgm[0, j, k] = sgm[0, j, k + 1] = 1.0
+    bp = recon.backproject_ir(sgm, g, dtype=np.float64)[0]
+    beta = float(spec.angles[j])
+    u = spec.u.astype(np.float64)
+    xs, ys = _image_xy(nx, pix)
+    X, Y = np.meshgrid(xs, ys)  # [row][col]
+    up = _project_point(X, Y, beta, sid, sdd)
+    r_fov = sid * np.sin(np.arctan(np.abs(u).max() / sdd))
+    rad = np.hypot(X, Y)
+    lo, hi = min(u[k], u[k + 1]), max(u[k], u[k + 1])
+    inside = (up > lo + 0.05 * w) & (up < hi - 0.05 * w) & (rad < r_fov - 2 * pix)
+    dxs, dys = X - sid * np.cos(beta), Y - sid * np.sin(beta)
+    ang = np.arctan2(dys, dxs)
+    chord = pix / np.maximum(np.abs(np.cos(ang)), np.abs(np.sin(ang)))
+    assert inside.sum() > 20
+    assert np.allclose(bp[inside], chord[inside], rtol=1e-9)
+    outside = (up < min(u[k - 1], u[k + 2]) - 0.05 * w) | (up > max(u[k - 1], u[k + 2]) + 0.05 * w)
+    assert np.all(bp[outside] == 0)
+    ones = recon.backproject_ir(np.ones((1, 8, 120)), g, dtype=np.float64)[0]
+    ups = np.stack([_project_point(X, Y, b, sid, sdd) for b in spec.angles.astype(np.float64)])
+    # the range test keeps (u - u[0]) / du in [0, n_u - 2] (run_mgfbp_iterative_
+    # recon_IRN_CGLS.py:439-441): the interval between the last two elements
+    # counts as off the detector
+    ulo, uhi = sorted((u[0], u[-2]))
+    margin = np.minimum(ups - ulo, uhi - ups).min(axis=0)  # < 0: off the detector
+    assert np.all(ones[margin < -0.1 * w] == 0)
+    assert np.all(ones[margin > 0.1 * w] > 0)
+    assert (margin < -0.1 * w).sum() > 100
+
+
+def test_ray_mask_matches_cylinder_intersection():
+    """GenEffectiveMapForwardProjectionAgent's cone-beam map against exact
+    geometry: a ray is masked iff its segment inside the image's inscribed
+    cylinder leaves the z range between the first and last slice centres.
+    Rays within a pixel (in-plane) or a slice (in z) of either boundary are
+    not classified."""
+    nx, nz, pix, vh, h = 32, 8, 0.5, 0.5, 0.5
+    sid, sdd = 150.0, 300.0
+    spec = cone_case(nx=nx, nz=nz, nv=6, nu=60, nrows=40, pix=pix, vh=vh, w=0.5, h=h, sid=sid,
+                     sdd=sdd)
+    g = oracle_dict(spec)
+    m = recon.ray_mask(g, dtype=np.float64)
+    R = nx * pix / 2
+    z_lo, z_hi = -(nz - 1) / 2 * vh, (nz - 1) / 2 * vh
+    u = spec.u.astype(np.float64)
+    v = spec.v.astype(np.float64)
+    # gantry frame: source (sid, 0, 0), element (sid - sdd, u, v); the cylinder
+    # is centred on the rotation axis, so the view angle does not matter
+    U, V = np.meshgrid(u, v)  # [row][col]
+    dx, dy = -sdd, U
+    a = dx * dx + dy * dy
+    b = 2 * sid * dx
+    def zrange(rad):
+        c = sid * sid - rad * rad
+        disc = b * b - 4 * a * c
+        hit = disc > 0
+        sq = np.sqrt(np.where(hit, disc, 0))
+        t1, t2 = (-b - sq) / (2 * a), (-b + sq) / (2 * a)
+        z1, z2 = t1 * V, t2 * V
+        return hit, np.minimum(z1, z2), np.maximum(z1, z2)
+    def kept(rad, dz):  # dz > 0 widens the allowed z range
+        hit, zmin, zmax = zrange(rad)
+        return ~(hit & ((zmin < z_lo - dz) | (zmax > z_hi + dz)))
+
+    # robust: the same verdict for the smaller cylinder with the wider z range
+    # and for the larger cylinder with the narrower one
+    small_wide, big_narrow = kept(R - 2 * pix, vh), kept(R + 2 * pix, -vh)
+    sure = small_wide == big_narrow
+    want = small_wide
+    assert sure.mean() > 0.6 and want[sure].any() and (~want[sure]).any()
+    for jv in range(spec.n_views):
+        assert np.array_equal(m[:, jv][sure] == 1, want[sure]), jv
 
 
 # ------------------------------------------------------------------ RNG
