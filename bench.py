@@ -28,23 +28,29 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per back-projection launch of this workload, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
 # (FETCH x2, WRITE x1); profiles/fdk_c1_kernels.md.  Keyed by
-# (storage, --bp-tile, --bp-batch); other settings report null.
-MEASURED_TRAFFIC = {("f32", 0, 0): 94.2e9}
-# Vector L1 fill rate of a CDNA CU: 64 B/clk; MI355X peak engine clock 2.4 GHz.
-L1_BYTES_PER_CLK_PER_CU = 64
+# (kernel, storage, --bp-tile, --bp-batch); other settings report null.
+MEASURED_TRAFFIC = {("direct", "f32", 0, 0): 94.2e9}
+# Per-CU fill rates on CDNA: vector L1 64 B/clk, LDS 128 B/clk; MI355X peak
+# engine clock 2.4 GHz.
+GATHER_BYTES_PER_CLK_PER_CU = {"direct": 64, "lds": 128}
+GATHER_PATH = {"direct": "vector L1", "lds": "LDS"}
+KERNEL_NAME = {"direct": "bp_cone_axial_kernel", "lds": "bp_cone_lds_kernel"}
 CLOCK_GHZ = 2.4
 
 
-def l1_roofline(gather_bytes: float, ms: float) -> dict:
-    """The back-projector's taps arrive through the per-CU vector L1 (one
-    16-byte FP32 or 8-byte FP16/BF16 gather per update): its ceiling."""
+def gather_roofline(kernel: str, gather_bytes: float, ms: float) -> dict:
+    """Every update reads its four taps (16 B FP32, 8 B FP16/BF16) through the
+    vector L1 (direct kernel) or from LDS (staged kernel): that path's
+    ceiling."""
     import torch
 
+    if kernel not in GATHER_PATH:
+        return None
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    peak = cus * L1_BYTES_PER_CLK_PER_CU * CLOCK_GHZ  # GB/s
+    peak = cus * GATHER_BYTES_PER_CLK_PER_CU[kernel] * CLOCK_GHZ  # GB/s
     ach = gather_bytes / (ms * 1e-3) / 1e9
-    return {"achieved": round(ach, 1), "peak": round(peak, 1), "unit": "GB/s",
-            "frac": round(ach / peak, 4), "cus": cus}
+    return {"path": GATHER_PATH[kernel], "achieved": round(ach, 1), "peak": round(peak, 1),
+            "unit": "GB/s", "frac": round(ach / peak, 4), "cus": cus}
 
 
 def c1_config(storage: str, filter_method: str) -> dict:
@@ -99,6 +105,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bp-tile", type=int, default=0, help="slices per thread (0: library default)")
     ap.add_argument("--bp-batch", type=int, default=0, help="loads per batch (0: library default)")
+    ap.add_argument("--bp-kernel", default="auto", choices=["auto", "direct", "lds"],
+                    help="cone back-projector: LDS-staged windows or per-update gathers")
     ap.add_argument("--sinogram", default="phantom", choices=["phantom", "random"],
                     help="analytic ellipsoid projections, or seeded uniform noise (profiler runs)")
     ap.add_argument("--workload", default="fdk", choices=["fdk", "fbp2d", "fp3d", "ir"],
@@ -149,6 +157,7 @@ def main() -> int:
     plan = fbp.plan()
     if args.bp_tile or args.bp_batch:
         fbp.ctx().set_bp_tile(args.bp_tile, args.bp_batch)
+    fbp.ctx().set_bp_kernel(args.bp_kernel)
     stream = torch.cuda.current_stream(dev)
 
     bp_times = []
@@ -185,6 +194,7 @@ def main() -> int:
     tap_bytes = 4 if args.storage == "f32" else 2
     bytes_alg = U * 4 * tap_bytes + float(nx) * nx * nz * 4
     achieved = bytes_alg / (bp_ms * 1e-3) / 1e9
+    kernel = fbp.ctx().bp_last_kernel()
     line = {
         "metric": "FDK/backprojection GUPS", "value": round(gups, 2), "unit": "GUPS",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -196,11 +206,12 @@ def main() -> int:
                    "bp_ms": round(bp_ms, 3), "bp_gups": round(U / (bp_ms * 1e-3) / 1e9, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((args.storage, args.bp_tile, args.bp_batch)),
-                     "kernel": "bp_cone_axial_kernel",
+                     "traffic": MEASURED_TRAFFIC.get((kernel, args.storage, args.bp_tile,
+                                                      args.bp_batch)),
+                     "kernel": KERNEL_NAME.get(kernel, kernel),
                      "model": f"gather bytes: 4 taps x {tap_bytes} B per update + 4 B per voxel; "
-                              "the taps are served by L1/L2, so frac > 1 is expected",
-                     "l1_gather": l1_roofline(U * 4 * tap_bytes, bp_ms)},
+                              "the taps are served by LDS/L1/L2, so frac > 1 is expected",
+                     "gather": gather_roofline(kernel, U * 4 * tap_bytes, bp_ms)},
     }
     if not args.no_cpu:
         filt_host = fbp.last_filtered.cpu().numpy()

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 2
+#define MCT_ABI_VERSION 3
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -188,8 +188,26 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs);
 
 /* Tile shape of the axial back-projector (autotuner hook): zc slices per
  * thread (4, 8, 16 or 32) and, for cone beam, batch taps loads issued
- * together (4, 8 or 16, at most zc); 0 restores a default. */
+ * together (4, 8 or 16, at most zc); 0 restores a default.  Applies to the
+ * direct-gather kernel. */
 int mct_set_bp_tile(mct_ctx *ctx, int32_t zc, int32_t batch);
+
+/* Cone-beam axial back-projector used by mct_backproject_fbp (QUAD taps):
+ *   MCT_BP_AUTO   the LDS-staged kernel where the geometry allows it
+ *                 (flat or curved detector, no P-matrix, detector windows
+ *                 that fit in LDS), else the direct-gather kernel;
+ *   MCT_BP_DIRECT every voxel-view update gathers its taps from memory;
+ *   MCT_BP_LDS    each 16 x 16-column x 16-slice tile stages the detector
+ *                 window it reads in a view into LDS once (MCT_ERR_UNSUPPORTED
+ *                 where the geometry does not allow it).
+ * The LDS-staged kernel builds two geometry tables in the plan on its first
+ * use (one synchronisation of the context's stream). */
+typedef enum mct_bp_kernel { MCT_BP_AUTO = 0, MCT_BP_DIRECT = 1, MCT_BP_LDS = 2 } mct_bp_kernel;
+int mct_set_bp_kernel(mct_ctx *ctx, int32_t kind);
+/* Kernel the last mct_backproject_fbp call on ctx launched: MCT_BP_DIRECT,
+ * MCT_BP_LDS, 3 (fan-beam axial), 4 (generic: other view modes, P-matrix,
+ * PLAIN taps) or 0 (none yet); -1 for a NULL ctx. */
+int mct_bp_last_kernel(const mct_ctx *ctx);
 
 /* ---- forward projection -------------------------------------------------- */
 

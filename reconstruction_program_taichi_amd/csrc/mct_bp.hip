@@ -38,6 +38,13 @@ namespace {
 // 32-bit untyped data, no swizzle.
 constexpr int kBufferDword3 = 0x00020000;
 
+// Window of packed cells of one (tile, chunk, view) for the LDS-staged cone
+// kernel: first cell (column c0, row r0) and extent w x h; all inside
+// [0, n_u-2] x [0, n_v-2].
+struct BpWin {
+    int16_t c0, r0, w, h;
+};
+
 struct BpArgs {
     int nx, nz, nu, nv, nviews;
     float pix, vh, cx, cy, cz, cos_rot, sin_rot;
@@ -52,6 +59,11 @@ struct BpArgs {
     int layout;
     float *vol;
     int tiles_x, tiles_y, nzc, nxcd;
+    // LDS-staged cone kernel: the plan's tables and the LDS window shape
+    // (win == NULL: the direct kernel runs this job)
+    const BpWin *win;
+    const int2 *iv;
+    int pitch, rows;
 };
 
 template <int DT> struct Taps;
@@ -296,6 +308,253 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
     cone_tile<ZC, B, DT>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd));
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged cone back-projector (axial view, flat or curved detector, QUAD
+// taps).  A workgroup owns a 16 x 16-column tile and a chunk of 16 slices
+// (one column and all 16 slices per thread).  In one view the tile's voxels
+// read packed cells from a small window of the detector; the window of every
+// (tile, chunk, view) is tabulated once per plan from the tile's corners: a
+// voxel's detector coordinates are linear-fractional in its position (and the
+// row coordinate linear in z), so their extremes over the tile are taken at
+// the corners, and one cell of margin on each side covers FP32 rounding.  The
+// workgroup copies the window of view j+1 into one LDS buffer while it
+// interpolates view j from the other, so a packed cell crosses the vector
+// memory path once per tile and view instead of once per voxel.  The mask (a
+// voxel that leaves the detector in any view is 0) comes from the plan's
+// per-column slice intervals, decided with the same exact tests as the
+// direct kernel.
+
+constexpr int kLdsTile = 16;  // columns per side of a workgroup tile (decode_tile)
+constexpr int kLdsZC = 16;    // slices per thread
+
+// d = fl(fl(mag * z) - v[0]) of slice iz, the quantity of the exact v test.
+__device__ __forceinline__ float v_coord(const BpArgs &a, float mag, int iz, float halfz) {
+    float z = ((float)iz - halfz) * a.vh + a.cz;
+    return mag * z - a.v0;
+}
+
+// Per image column: the slices that pass the range tests of every view,
+// [lo, hi] (lo > hi: none).  d(iz) is non-decreasing in iz (mag > 0, vh > 0),
+// so each view's passing slices form an interval whose ends are found by
+// bisection with the exact test.
+__global__ __launch_bounds__(256) void bp_column_iv_kernel(BpArgs a, int2 *__restrict__ iv) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y;
+    if (ix >= a.nx) return;
+    float x, y;
+    voxel_xy(a, ix, iy, x, y);
+    const float halfz = (float)(a.nz - 1) / 2.0f;
+    int lo = 0, hi = a.nz - 1;
+    for (int j = 0; j < a.nviews && lo <= hi; ++j) {
+        const ColumnView cv = column_view(a, x, y, a.vt[j]);
+        if (!cv.ok) {
+            lo = 1;
+            hi = 0;
+            break;
+        }
+        int l = lo, r = hi + 1;  // first slice with d >= da
+        while (l < r) {
+            int m = (l + r) >> 1;
+            if (v_coord(a, cv.mag, m, halfz) >= a.da)
+                r = m;
+            else
+                l = m + 1;
+        }
+        const int nlo = l;
+        l = lo;
+        r = hi + 1;  // first slice with d > db
+        while (l < r) {
+            int m = (l + r) >> 1;
+            if (v_coord(a, cv.mag, m, halfz) > a.db)
+                r = m;
+            else
+                l = m + 1;
+        }
+        lo = nlo;
+        hi = l - 1;
+    }
+    iv[(size_t)iy * a.nx + ix] = make_int2(lo, hi);
+}
+
+// Window table, FP64 corner evaluation; maxes[0..1] collect the largest w, h.
+// A tile with a corner at or behind the source (L <= 0) gets an oversized
+// window, which makes the launch fall back to the direct kernel.
+__global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restrict__ win,
+                                                       int *__restrict__ maxes) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nviews) return;
+    const int tile = blockIdx.y, zci = blockIdx.z;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int ixs[2] = {tx * kLdsTile, min(tx * kLdsTile + kLdsTile - 1, a.nx - 1)};
+    const int iys[2] = {ty * kLdsTile, min(ty * kLdsTile + kLdsTile - 1, a.nx - 1)};
+    const int iz0 = zci * kLdsZC, iz1 = min(iz0 + kLdsZC - 1, a.nz - 1);
+    const double half = (a.nx - 1) / 2.0, halfz = (a.nz - 1) / 2.0;
+    const double c = a.vt[j].x, s = a.vt[j].y;
+    const double zs[2] = {(iz0 - halfz) * a.vh + a.cz, (iz1 - halfz) * a.vh + a.cz};
+    double umin = 1e300, umax = -1e300, tmin = 1e300, tmax = -1e300;
+    bool bad = false;
+    for (int p = 0; p < 2; ++p) {
+        for (int q = 0; q < 2; ++q) {
+            const double xa = (double)a.pix * (ixs[p] - half) + a.cx;
+            const double ya = -(double)a.pix * (iys[q] - half) + a.cy;
+            const double x = xa * a.cos_rot + ya * a.sin_rot;
+            const double y = -xa * a.sin_rot + ya * a.cos_rot;
+            const double L = a.sid - x * c - y * s;
+            if (!(L > 0.0)) bad = true;
+            const double yr = -x * s + y * c;
+            const double mag = a.sdd / L;
+            const double u = a.curved ? a.sdd * atan2(yr, L) : mag * yr;
+            const double ui = (u - a.u0) * a.inv_du;
+            umin = fmin(umin, ui);
+            umax = fmax(umax, ui);
+            for (int r = 0; r < 2; ++r) {
+                const double t = (mag * zs[r] - a.v0) * a.vinv;
+                tmin = fmin(tmin, t);
+                tmax = fmax(tmax, t);
+            }
+        }
+    }
+    const double cmax = a.nu - 2, rmax = a.nv - 2;
+    const double cl = fmin(fmax(floor(umin) - 1.0, 0.0), cmax);
+    const double ch = fmin(fmax(floor(umax) + 1.0, 0.0), cmax);
+    const double rl = fmin(fmax(floor(tmin) - 1.0, 0.0), rmax);
+    const double rh = fmin(fmax(floor(tmax) + 1.0, 0.0), rmax);
+    int w = ch >= cl ? (int)(ch - cl) + 1 : 0;
+    int h = rh >= rl ? (int)(rh - rl) + 1 : 0;
+    if (bad || !(umin <= umax) || !(tmin <= tmax)) w = h = 32767;
+    BpWin e;
+    e.c0 = (int16_t)cl;
+    e.r0 = (int16_t)rl;
+    e.w = (int16_t)w;
+    e.h = (int16_t)h;
+    win[((size_t)tile * a.nzc + zci) * a.nviews + j] = e;
+    atomicMax(&maxes[0], w);
+    atomicMax(&maxes[1], h);
+}
+
+// One workgroup tile (tile, chunk) = block of job a.  a.pitch x a.rows: the
+// LDS window shape (largest w and h of the job's table); NQ: staging slots
+// per thread, NQ * 256 >= pitch * rows; smem: two buffers of pitch * rows
+// packed cells.  Every thread of the workgroup must call this (barriers).
+template <int DT, int NQ>
+__device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsigned char *smem) {
+    using T = Taps<DT>;
+    using Raw = typename T::Raw;
+    Raw *lds = reinterpret_cast<Raw *>(smem);
+    const int pitch = a.pitch, rows = a.rows;
+    const BpWin *__restrict__ win = a.win;
+    const int2 *__restrict__ iv = a.iv;
+    const int cells = pitch * rows;
+    int ix, iy, zci;
+    decode_tile(a, block, ix, iy, zci);
+    const int tile = block % (a.tiles_x * a.tiles_y);
+    const bool inside = ix < a.nx && iy < a.nx;
+    const int z0 = zci * kLdsZC;
+    const int nk = a.nz - z0 < kLdsZC ? a.nz - z0 : kLdsZC;
+    int lo = 0, hi = -1;
+    if (inside) {
+        const int2 v = iv[(size_t)iy * a.nx + ix];
+        lo = v.x - z0 > 0 ? v.x - z0 : 0;
+        hi = v.y - z0 < nk - 1 ? v.y - z0 : nk - 1;
+    }
+    const bool alive = lo <= hi;
+    float acc[kLdsZC];
+#pragma unroll
+    for (int k = 0; k < kLdsZC; ++k) acc[k] = 0.f;
+
+    if (__syncthreads_or(alive)) {
+        const int tid = threadIdx.x;
+        const uint32_t rowbytes = (uint32_t)(a.nu - 1) * T::kQuadBytes;
+        const size_t viewbytes = (size_t)(a.nv - 1) * rowbytes;
+        const char *sino = reinterpret_cast<const char *>(a.sino);
+        // staging slot q of this thread: window entry e = tid + 256 q, i.e.
+        // (row e / pitch, column e % pitch); its byte offset from the window
+        // origin is the same in every view
+        int srow[NQ];
+        uint32_t soff[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / pitch, c = e - r * pitch;
+            srow[q] = e < cells ? r : 0x7fffffff;
+            soff[q] = (uint32_t)r * rowbytes + (uint32_t)c * T::kQuadBytes;
+        }
+        const BpWin *wt = win + ((size_t)tile * a.nzc + zci) * a.nviews;
+        Raw stage[NQ];
+        auto fetch = [&](int j, const BpWin &w) {
+            const size_t origin = (size_t)w.r0 * rowbytes + (size_t)w.c0 * T::kQuadBytes;
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(sino + (size_t)j * viewbytes + origin), 0, (int)(viewbytes - origin),
+                kBufferDword3);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                if (srow[q] < w.h) stage[q] = T::load_buf(rsrc, soff[q]);
+        };
+        auto put = [&](int b, const BpWin &w) {
+            Raw *dst = lds + b * cells + tid;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                if (srow[q] < w.h) dst[256 * q] = stage[q];
+        };
+        float x, y;
+        voxel_xy(a, ix, iy, x, y);
+        const float halfz = (float)(a.nz - 1) / 2.0f;
+        const float zfirst = ((float)z0 - halfz) * a.vh + a.cz;
+        const float zstep_t = a.vh * a.vinv;
+
+        BpWin wc = wt[0];
+        fetch(0, wc);
+        put(0, wc);
+        __syncthreads();
+        for (int j = 0; j < a.nviews; ++j) {
+            const bool more = j + 1 < a.nviews;
+            BpWin wn = wc;
+            if (more) {
+                wn = wt[j + 1];
+                fetch(j + 1, wn);
+            }
+            if (alive) {
+                const ColumnView cv = column_view(a, x, y, a.vt[j]);
+                // row coordinate relative to the window's first row; the
+                // subtraction is exact when 0 <= r0 <= t0, which holds
+                // whenever a slice of the chunk passes the range tests
+                const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - (float)wc.r0;
+                const float dt = cv.mag * zstep_t;
+                int lc = (int)cv.uf - wc.c0;
+                const int wmax = wc.w > 0 ? wc.w - 1 : 0;
+                lc = lc < 0 ? 0 : (lc > wmax ? wmax : lc);
+                // largest float below h: a clamped row coordinate floors to <= h - 1
+                const float tlim = wc.h > 0 ? __uint_as_float(__float_as_uint((float)wc.h) - 1u) : 0.f;
+                const Raw *src = lds + (j & 1) * cells + lc;
+#pragma unroll
+                for (int k = 0; k < kLdsZC; ++k) {
+                    float tk = fmaf((float)k, dt, t0);
+                    tk = __builtin_amdgcn_fmed3f(tk, 0.f, tlim);
+                    const uint32_t vf = (uint32_t)tk;
+                    const float rv = __builtin_amdgcn_fractf(tk);
+                    acc[k] = fmaf(cv.coef, bilerp(T::cvt(src[vf * (uint32_t)pitch]), cv.ru, rv), acc[k]);
+                }
+            }
+            if (more) put((j + 1) & 1, wn);
+            __syncthreads();
+            wc = wn;
+        }
+    }
+    if (inside) {
+        const size_t plane = (size_t)a.nx * a.nx;
+        float *out = a.vol + (size_t)iy * a.nx + ix;
+#pragma unroll
+        for (int k = 0; k < kLdsZC; ++k)
+            if (k < nk) out[(size_t)(z0 + k) * plane] = (k >= lo && k <= hi) ? acc[k] : 0.f;
+    }
+}
+
+template <int DT, int NQ>
+__global__ __launch_bounds__(256) void bp_cone_lds_kernel(BpArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cone_lds_tile<DT, NQ>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
+                          smem);
+}
+
 // Fan beam, axial view, PAIR taps; slice z reads detector row z.
 template <int ZC, int DT>
 __device__ __forceinline__ void fan_tile(const BpArgs &a, int block) {
@@ -345,11 +604,15 @@ __global__ __launch_bounds__(256) void bp_fan_axial_kernel(BpArgs a) {
 // are numbered job after job; offsets[j] is the first item of job j
 // (offsets[njobs] = total).  A fixed grid walks the items in a static round
 // robin, so the launch is deterministic and needs no counters (it can be
-// captured in a graph and replayed).
-template <int DT>
+// captured in a graph and replayed).  Cone jobs with window tables run the
+// LDS-staged tile (NQ staging slots; NQ = 0: none in this batch), the others
+// the direct ones.  An item's job is uniform over the workgroup, so the
+// staged tile's barriers are reached by every thread.
+template <int DT, int NQ>
 __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict__ jobs,
                                                        const int *__restrict__ offsets, int njobs,
                                                        int total) {
+    extern __shared__ __align__(16) unsigned char smem[];
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = njobs - 1;  // last job with offsets[job] <= item
         while (lo < hi) {
@@ -361,6 +624,12 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict_
         }
         const BpArgs a = jobs[lo];
         const int local = item - offsets[lo];
+        if constexpr (NQ > 0) {
+            if (a.cone && a.win) {
+                cone_lds_tile<DT, NQ>(a, local, smem);
+                continue;
+            }
+        }
         if (a.cone)
             cone_tile<16, 16, DT>(a, local);
         else
@@ -611,6 +880,114 @@ BpArgs make_args(const mct_plan *p) {
 #define MCT_AXIAL_LAUNCH(KERNEL, ZC, DT) \
     hipLaunchKernelGGL((KERNEL<ZC, DT>), dim3(nblocks), dim3(256), 0, ctx->stream, a)
 
+// Window tables larger than this are not built (the direct kernel runs).
+constexpr size_t kMaxWinTableBytes = size_t(2) << 30;
+// Largest LDS window (cells) the staged kernel takes: 8 staging slots per
+// thread; two FP32 buffers of 2048 cells fill 64 KiB.
+constexpr int kLdsMaxCells = 2048;
+
+// Builds the plan's column intervals and window table on first use; one
+// synchronisation of the context's stream (to read the largest window).
+int ensure_lds_tables(mct_plan *p, BpArgs a) {
+    if (p->bp_win_state != 0) return p->bp_win_state > 0 ? MCT_OK : MCT_ERR_UNSUPPORTED;
+    mct_ctx *ctx = p->ctx;
+    p->bp_win_state = -1;
+    const size_t entries = (size_t)a.tiles_x * a.tiles_y * a.nzc * a.nviews;
+    if (entries * sizeof(BpWin) > kMaxWinTableBytes || a.tiles_x * a.tiles_y > 65535 ||
+        a.nzc > 65535)
+        return MCT_ERR_UNSUPPORTED;
+    MCT_HIP(hipMalloc(&p->bp_iv, sizeof(int2) * (size_t)a.nx * a.nx));
+    MCT_HIP(hipMalloc(&p->bp_win, sizeof(BpWin) * entries));
+    int *maxes = nullptr;
+    MCT_HIP(hipMalloc(&maxes, 2 * sizeof(int)));
+    int host[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(maxes, 0, 2 * sizeof(int), ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bp_column_iv_kernel, dim3(mct::blocks_for(a.nx, 256), a.nx), dim3(256), 0,
+                           ctx->stream, a, p->bp_iv);
+        hipLaunchKernelGGL(bp_window_kernel,
+                           dim3(mct::blocks_for(a.nviews, 64), a.tiles_x * a.tiles_y, a.nzc), dim3(64),
+                           0, ctx->stream, a, reinterpret_cast<BpWin *>(p->bp_win), maxes);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(host, maxes, sizeof(host), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(maxes);
+    if (e != hipSuccess)
+        return mct::set_error(MCT_ERR_DEVICE, "back-projector tables: %s", hipGetErrorString(e));
+    p->bp_win_w = host[0] > 0 ? host[0] : 1;
+    p->bp_win_h = host[1] > 0 ? host[1] : 1;
+    if ((int64_t)p->bp_win_w * p->bp_win_h > kLdsMaxCells) {
+        hipFree(p->bp_iv);
+        hipFree(p->bp_win);
+        p->bp_iv = nullptr;
+        p->bp_win = nullptr;
+        return MCT_ERR_UNSUPPORTED;
+    }
+    p->bp_win_state = 1;
+    return MCT_OK;
+}
+
+// Staging slots per thread for a window of `cells` packed cells (the
+// instantiated NQ values) and the dynamic LDS of its two buffers.
+inline int lds_slots(int cells) {
+    const int nq = (cells + 255) / 256;
+    return nq <= 2 ? 2 : nq <= 4 ? 4 : nq <= 6 ? 6 : 8;
+}
+inline size_t lds_bytes(int cells, int32_t dtype) {
+    return 2 * (size_t)cells * (dtype == MCT_F32 ? 16 : 8);
+}
+// Automatic selection declines windows whose two LDS buffers exceed this
+// (occupancy would fall below 4 workgroups per CU).
+constexpr size_t kLdsAutoBytes = 40 * 1024;
+
+// Fills a.win / a.iv / a.pitch / a.rows (and the 16 x 16 x 16 tiling) for the
+// LDS-staged kernel; MCT_ERR_UNSUPPORTED when the geometry does not allow it
+// (the caller then runs the direct kernel or reports it).
+int prepare_cone_lds(mct_plan *p, BpArgs &a, int32_t dtype, bool automatic) {
+    const mct_geometry &g = p->g;
+    if (!(g.cone && g.view_mode == 1 && !g.apply_pmatrix && a.vinv != 0.f && g.vox_h > 0.f &&
+          g.n_u >= 2 && g.n_v >= 2))
+        return MCT_ERR_UNSUPPORTED;
+    a.tiles_x = (g.img_dim + kLdsTile - 1) / kLdsTile;
+    a.tiles_y = a.tiles_x;
+    a.nzc = (g.img_dim_z + kLdsZC - 1) / kLdsZC;
+    int rc = ensure_lds_tables(p, a);
+    if (rc) return rc;
+    if (automatic && lds_bytes(p->bp_win_w * p->bp_win_h, dtype) > kLdsAutoBytes)
+        return MCT_ERR_UNSUPPORTED;
+    a.win = reinterpret_cast<const BpWin *>(p->bp_win);
+    a.iv = p->bp_iv;
+    a.pitch = p->bp_win_w;
+    a.rows = p->bp_win_h;
+    return MCT_OK;
+}
+
+int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
+    int rc = prepare_cone_lds(p, a, dtype, automatic);
+    if (rc) return rc;
+    const int cells = a.pitch * a.rows;
+    const size_t shmem = lds_bytes(cells, dtype);
+    const int nblocks = a.tiles_x * a.tiles_y * a.nzc;
+    mct_ctx *ctx = p->ctx;
+#define LDS_LAUNCH(DT, NQ) \
+    hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ>), dim3(nblocks), dim3(256), shmem, ctx->stream, a)
+#define LDS_DT(NQ)                                         \
+    if (dtype == MCT_F32) LDS_LAUNCH(MCT_F32, NQ);         \
+    else if (dtype == MCT_F16) LDS_LAUNCH(MCT_F16, NQ);    \
+    else LDS_LAUNCH(MCT_BF16, NQ);
+    switch (lds_slots(cells)) {
+        case 2: LDS_DT(2) break;
+        case 4: LDS_DT(4) break;
+        case 6: LDS_DT(6) break;
+        default: LDS_DT(8) break;
+    }
+#undef LDS_DT
+#undef LDS_LAUNCH
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -636,12 +1013,25 @@ int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t
     a.layout = layout;
     a.vol = vol;
     const bool fast = g.view_mode == 1 && !g.apply_pmatrix && layout != MCT_PLAIN;
+    if (g.cone && layout == MCT_QUAD && ctx->bp_kernel != MCT_BP_DIRECT) {
+        const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
+        int rc = launch_cone_lds(plan, a, dtype, automatic);
+        if (rc == MCT_OK) {
+            ctx->bp_last = MCT_BP_LDS;
+            return MCT_OK;
+        }
+        if (rc != MCT_ERR_UNSUPPORTED) return rc;
+        if (!automatic)
+            return mct::set_error(MCT_ERR_UNSUPPORTED,
+                                  "LDS-staged back-projection does not fit this geometry");
+    }
     if (fast) {
         int zc = ctx->bp_zc ? ctx->bp_zc : (g.cone ? 16 : 8);
         a.tiles_x = (g.img_dim + 15) / 16;
         a.tiles_y = a.tiles_x;
         a.nzc = (g.img_dim_z + zc - 1) / zc;
         int nblocks = a.tiles_x * a.tiles_y * a.nzc;
+        ctx->bp_last = g.cone ? MCT_BP_DIRECT : 3;
         if (g.cone) {
             int bt = ctx->bp_batch ? ctx->bp_batch : 16;
             bt = bt > zc ? zc : bt;
@@ -681,6 +1071,7 @@ int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t
     } else {
         int64_t n = (int64_t)g.img_dim_z * g.img_dim * g.img_dim;
         unsigned nblocks = mct::blocks_for(n, 256);
+        ctx->bp_last = 4;
         if (dtype == MCT_F32)
             hipLaunchKernelGGL(bp_generic_kernel<MCT_F32>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
         else if (dtype == MCT_F16)
@@ -744,6 +1135,7 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     BpArgs *args = reinterpret_cast<BpArgs *>(ctx->batch_host);
     int *offs = reinterpret_cast<int *>(reinterpret_cast<char *>(ctx->batch_host) + args_bytes);
     int64_t total = 0;
+    int lds_cells = 0;  // largest LDS window among the staged cone jobs
     for (int i = 0; i < njobs; ++i) {
         const mct_bp_job &jb = jobs[i];
         MCT_CHECK(jb.plan && jb.filt && jb.vol, "job %d: NULL plan, sinogram or volume", i);
@@ -763,6 +1155,19 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
         a.tiles_x = (g.img_dim + 15) / 16;
         a.tiles_y = a.tiles_x;
         a.nzc = (g.img_dim_z + zc - 1) / zc;
+        if (g.cone && ctx->bp_kernel != MCT_BP_DIRECT) {
+            const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
+            int rc = prepare_cone_lds(jb.plan, a, dtype, automatic);
+            if (rc == MCT_OK) {
+                const int cells = a.pitch * a.rows;
+                lds_cells = cells > lds_cells ? cells : lds_cells;
+            } else if (rc != MCT_ERR_UNSUPPORTED) {
+                return rc;
+            } else if (!automatic) {
+                return mct::set_error(MCT_ERR_UNSUPPORTED,
+                                      "job %d: LDS-staged back-projection does not fit its geometry", i);
+            }
+        }
         args[i] = a;
         offs[i] = (int)total;
         total += (int64_t)a.tiles_x * a.tiles_y * a.nzc;
@@ -773,17 +1178,35 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     MCT_HIP(hipEventRecord(ctx->batch_ev, ctx->stream));
     const BpArgs *dargs = reinterpret_cast<const BpArgs *>(ctx->batch_dev);
     const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(ctx->batch_dev) + args_bytes);
-    int64_t grid = (int64_t)ctx->cu_count * 8;
-    grid = grid < total ? grid : total;
-    if (dtype == MCT_F32)
-        hipLaunchKernelGGL(bp_batch_kernel<MCT_F32>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, dargs,
-                           doffs, njobs, (int)total);
-    else if (dtype == MCT_F16)
-        hipLaunchKernelGGL(bp_batch_kernel<MCT_F16>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, dargs,
-                           doffs, njobs, (int)total);
-    else
-        hipLaunchKernelGGL(bp_batch_kernel<MCT_BF16>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, dargs,
-                           doffs, njobs, (int)total);
+    const size_t shmem = lds_cells ? lds_bytes(lds_cells, dtype) : 0;
+    const int nq = lds_cells ? lds_slots(lds_cells) : 0;
+    // a persistent grid: as many workgroups as stay resident together
+#define BATCH_KERNEL(DT, NQ) bp_batch_kernel<DT, NQ>
+#define BATCH_LAUNCH(DT, NQ)                                                                        \
+    do {                                                                                            \
+        int per_cu = 0;                                                                             \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BATCH_KERNEL(DT, NQ), 256, shmem) \
+                != hipSuccess || per_cu < 1)                                                        \
+            per_cu = 1;                                                                             \
+        int64_t grid = (int64_t)ctx->cu_count * per_cu;                                             \
+        grid = grid < total ? grid : total;                                                         \
+        hipLaunchKernelGGL(BATCH_KERNEL(DT, NQ), dim3((unsigned)grid), dim3(256), shmem,            \
+                           ctx->stream, dargs, doffs, njobs, (int)total);                           \
+    } while (0)
+#define BATCH_DT(NQ)                                         \
+    if (dtype == MCT_F32) BATCH_LAUNCH(MCT_F32, NQ);         \
+    else if (dtype == MCT_F16) BATCH_LAUNCH(MCT_F16, NQ);    \
+    else BATCH_LAUNCH(MCT_BF16, NQ);
+    switch (nq) {
+        case 0: BATCH_DT(0) break;
+        case 2: BATCH_DT(2) break;
+        case 4: BATCH_DT(4) break;
+        case 6: BATCH_DT(6) break;
+        default: BATCH_DT(8) break;
+    }
+#undef BATCH_DT
+#undef BATCH_LAUNCH
+#undef BATCH_KERNEL
     MCT_LAUNCHED();
     return MCT_OK;
 }

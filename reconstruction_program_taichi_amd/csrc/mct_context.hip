@@ -19,7 +19,8 @@ int upload(T **dst, const T *src, size_t n) {
 }
 
 void free_plan_arrays(mct_plan *p) {
-    void *ptrs[] = {p->angles, p->u, p->v, p->pmatrix, p->center_z, p->matrix_a, p->x_s, p->bp_view, p->fp_view};
+    void *ptrs[] = {p->angles,   p->u,        p->v,       p->pmatrix, p->center_z, p->matrix_a,
+                    p->x_s,      p->bp_view,  p->fp_view, p->bp_iv,   p->bp_win};
     for (void *q : ptrs)
         if (q) hipFree(q);
 }
@@ -200,6 +201,16 @@ int mct_set_bp_tile(mct_ctx *ctx, int32_t zc, int32_t batch) {
     ctx->bp_batch = batch;
     return MCT_OK;
 }
+
+int mct_set_bp_kernel(mct_ctx *ctx, int32_t kind) {
+    if (!ctx) return mct::set_error(MCT_ERR_INVALID, "ctx is NULL");
+    if (kind < MCT_BP_AUTO || kind > MCT_BP_LDS)
+        return mct::set_error(MCT_ERR_INVALID, "back-projector kind must be 0, 1 or 2 (got %d)", kind);
+    ctx->bp_kernel = kind;
+    return MCT_OK;
+}
+
+int mct_bp_last_kernel(const mct_ctx *ctx) { return ctx ? ctx->bp_last : -1; }
 
 int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
     if (!out) return mct::set_error(MCT_ERR_INVALID, "out is NULL");

@@ -14,6 +14,8 @@ struct mct_ctx {
     int xcd_count = 8;
     int bp_zc = 0;     // slices per thread in the axial back-projector (0: default)
     int bp_batch = 0;  // loads issued together by the cone back-projector (0: default)
+    int bp_kernel = 0; // cone back-projector: 0 automatic, 1 direct gather, 2 LDS-staged
+    int bp_last = 0;   // kernel of the last mct_backproject_fbp (mct_bp_last_kernel)
     // partial sums of the deterministic FP64 reductions
     double *partials = nullptr;
     int64_t partials_cap = 0;
@@ -56,6 +58,14 @@ struct mct_plan {
     // intervals: the u test passes iff ua <= u <= ub for u = fl(mag * yr);
     // the v test passes iff da <= d <= db for d = fl(fl(mag * z) - v[0]).
     float ua = 0.f, ub = 0.f, da = 0.f, db = 0.f;
+    // Tables of the LDS-staged cone back-projector, built on its first use
+    // (geometry only): per image column the exact interval of slices that
+    // pass every view's range tests, and per (16 x 16-column tile, 16-slice
+    // chunk, view) the window of packed detector cells the tile can read.
+    int2 *bp_iv = nullptr;
+    void *bp_win = nullptr;
+    int bp_win_w = 0, bp_win_h = 0;  // largest window over all entries
+    int bp_win_state = 0;            // 0 not built, 1 usable, -1 unusable geometry
 };
 
 namespace mct {

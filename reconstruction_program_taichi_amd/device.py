@@ -64,6 +64,18 @@ class Context:
     def set_bp_tile(self, zc: int, batch: int = 0) -> None:
         call("mct_set_bp_tile", self.h, int(zc), int(batch))
 
+    BP_KERNELS = {"auto": _abi.MCT_BP_AUTO, "direct": _abi.MCT_BP_DIRECT, "lds": _abi.MCT_BP_LDS}
+
+    def set_bp_kernel(self, kind: str) -> None:
+        """Cone-beam back-projector: 'auto', 'direct' (per-update gathers) or
+        'lds' (detector windows staged in LDS)."""
+        call("mct_set_bp_kernel", self.h, self.BP_KERNELS[kind])
+
+    def bp_last_kernel(self) -> str:
+        """Which back-projector the last FBP back-projection on this context ran."""
+        k = _abi.lib().mct_bp_last_kernel(self.h)
+        return {0: "none", 1: "direct", 2: "lds", 3: "fan", 4: "generic"}.get(k, str(k))
+
     def close(self) -> None:
         if getattr(self, "h", None):
             _abi.lib().mct_destroy(self.h)

@@ -160,15 +160,84 @@ def test_bp_fan_plain_generic(ctx):
 @pytest.mark.parametrize("zc,batch", [(4, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 4), (32, 8),
                                       (32, 16)])
 def test_bp_cone_quad_tiles(ctx, zc, batch):
+    """Direct-gather kernel at every tile shape."""
     spec = cone_case(nz=37)  # taller than the detector's cone: top/bottom slices masked
     ctx.set_bp_tile(zc, batch)
+    ctx.set_bp_kernel("direct")
     try:
         got, ref, alive, _ = _bp_check(ctx, spec, "quad")
     finally:
         ctx.set_bp_tile(0, 0)
+        ctx.set_bp_kernel("auto")
     assert 0.02 < (~alive).mean() < 0.98  # the detector-range mask is exercised
     assert mask_disagreements(got, ref, alive) == 0
     assert rel_l2(got, ref) <= 1e-5
+
+
+def _lds_spec(kind):
+    if kind == "tall":
+        return cone_case(nz=37)
+    if kind == "top_row_offset":
+        return cone_case(top_row=True, off_v=1.3, nz=30)
+    if kind == "odd_sizes":  # partial tiles in x/y and a partial slice chunk
+        return cone_case(nx=41, nz=21, nv=50, nu=83, nrows=29)
+    if kind == "rotated_offcentre":
+        spec = cone_case(nz=20)
+        from dataclasses import replace
+
+        from reconstruction_program_taichi_amd import geometry as geo
+        rot = 0.4
+        return replace(spec, img_rot=rot, cx=1.7, cy=-2.2,
+                       angles=geo.angle_array(spec.n_views, 2 * PI, rot))
+    if kind == "curved":
+        from dataclasses import replace
+        return replace(cone_case(nz=30), curved=True)
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["tall", "top_row_offset", "odd_sizes", "rotated_offcentre", "curved"])
+def test_bp_cone_lds_staged(ctx, kind):
+    """LDS-staged kernel against the oracle (identical masks) and against the
+    direct-gather kernel.  The two differ only in where the row coordinate is
+    rounded (relative to the window's first row, or to detector row 0), about
+    1e-6 rel-L2 on these white-noise sinograms."""
+    spec = _lds_spec(kind)
+    ctx.set_bp_kernel("lds")
+    try:
+        got, ref, alive, f = _bp_check(ctx, spec, "quad", seed=21)
+        assert ctx.bp_last_kernel() == "lds"
+        ctx.set_bp_kernel("direct")
+        direct, _, _, _ = _bp_check(ctx, spec, "quad", seed=21)
+        assert ctx.bp_last_kernel() == "direct"
+    finally:
+        ctx.set_bp_kernel("auto")
+    print(f"{kind}: lds vs oracle {rel_l2(got, ref):.2e}, direct vs oracle {rel_l2(direct, ref):.2e}, "
+          f"lds vs direct {rel_l2(got, direct):.2e}")
+    assert mask_disagreements(got, ref, alive) == 0
+    assert rel_l2(got, ref) <= 1e-5
+    assert np.array_equal(got != 0, direct != 0)
+    assert rel_l2(got, direct) <= 4e-6
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+def test_bp_cone_lds_half_storage_equals_direct(ctx, dtype):
+    spec = cone_case(nz=37)
+    rng = np.random.default_rng(13)
+    f = (1.0 + 0.3 * rng.standard_normal((spec.n_v, spec.n_views, spec.n_u))).astype(np.float32)
+    plan = Plan(ctx, spec)
+    packed = ctx.pack(dev(f), dtype, "quad")
+    out = {}
+    for kind in ("lds", "direct"):
+        ctx.set_bp_kernel(kind)
+        try:
+            vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32,
+                              device=DEV)
+            plan.backproject_fbp(packed, dtype, "quad", vol)
+            out[kind] = host(vol)
+        finally:
+            ctx.set_bp_kernel("auto")
+    assert np.array_equal(out["lds"] != 0, out["direct"] != 0)
+    assert rel_l2(out["lds"], out["direct"]) <= 1e-6
 
 
 @pytest.mark.parametrize("variant", ["top_row", "offset"])

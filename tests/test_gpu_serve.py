@@ -24,20 +24,29 @@ def _single(ctx, spec, f, dtype):
     return plan, packed, layout, vol
 
 
+@pytest.mark.parametrize("kernel", ["auto", "direct"])
 @pytest.mark.parametrize("dtype", ["f32", "f16"])
-def test_batched_bp_equals_single_jobs(ctx, dtype):
+def test_batched_bp_equals_single_jobs(ctx, dtype, kernel):
+    """The cone jobs run the LDS-staged tile (auto) or the direct one, in the
+    batch and alone alike."""
     specs = [fan_case(nx=48, nv=60, nu=80, rows=3), cone_case(nx=40, nz=21, nv=50),
              fan_case(nx=33, nv=45, nu=64, rows=1, scan_deg=200.0), cone_case(nx=56, nz=37, nv=64, top_row=True)]
     rng = np.random.default_rng(0)
     singles, jobs = [], []
-    for s in specs:
-        f = torch.from_numpy(rng.standard_normal((s.n_v, s.n_views, s.n_u)).astype(np.float32)).cuda()
-        plan, packed, layout, vol = _single(ctx, s, f, dtype)
-        singles.append(vol.clone())
-        out = torch.full_like(vol, 7.0)
-        jobs.append((plan, packed, dtype, layout, out))
-    ctx.backproject_batch(jobs)
-    torch.cuda.synchronize()
+    ctx.set_bp_kernel(kernel)
+    try:
+        for s in specs:
+            f = torch.from_numpy(rng.standard_normal((s.n_v, s.n_views, s.n_u)).astype(np.float32)).cuda()
+            plan, packed, layout, vol = _single(ctx, s, f, dtype)
+            if s.cone:
+                assert ctx.bp_last_kernel() == ("lds" if kernel == "auto" else "direct")
+            singles.append(vol.clone())
+            out = torch.full_like(vol, 7.0)
+            jobs.append((plan, packed, dtype, layout, out))
+        ctx.backproject_batch(jobs)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_bp_kernel("auto")
     for want, job in zip(singles, jobs):
         assert torch.equal(job[4], want)
 
