@@ -40,6 +40,16 @@ def test_hamming_taps_frequency_response(t):
     assert np.allclose(r, expect, rtol=5e-3, atol=2e-3 / (2 * w))
 
 
+def test_shepp_logan_taps_frequency_response():
+    """Shepp-Logan taps: the ramp times a sinc window, |sin(pi f w)| / (pi w)
+    on |f| < 1/(2w) (Shepp & Logan 1974)."""
+    n, w = 512, 0.4
+    h = geo.shepp_logan_taps(n, w).astype(np.float64)
+    f = np.linspace(0.05, 0.95, 19) / (2 * w)
+    r = _response(h, w, f)
+    assert np.allclose(r, np.abs(np.sin(np.pi * f * w)) / (np.pi * w), rtol=3e-3, atol=0)
+
+
 def test_hamming_t1_equals_ramlak_oracle():
     h32 = geo.hamming_taps(300, 0.4, 1.0, 1000.0)
     assert np.array_equal(h32, recon.ramlak_taps(300, 0.4, np.float32))
