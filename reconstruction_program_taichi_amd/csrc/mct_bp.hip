@@ -327,6 +327,38 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
 constexpr int kLdsTile = 16;  // columns per side of a workgroup tile (decode_tile)
 constexpr int kLdsZC = 16;    // slices per thread
 
+struct FastView {
+    uint32_t uf;
+    float ru, mag, coef;
+};
+
+// Per (column, view) interpolation geometry of run_mgfbp.py:1122-1155 for the
+// staged kernel.  The mask is decided by the plan's exact slice intervals,
+// so 1/L comes from the hardware reciprocal (1 ulp) instead of a division.
+__device__ __forceinline__ FastView column_view_fast(const BpArgs &a, float x, float y, float4 t) {
+    FastView cv;
+    const float L = a.sid - x * t.x - y * t.y;
+    const float rL = __builtin_amdgcn_rcpf(L);
+    cv.mag = a.sdd * rL;
+    const float yr = (-x) * t.y + y * t.x;
+    float u, w;
+    if (a.curved) {
+        u = a.sdd * atan2f(yr, L);
+        const float q = x * t.y - y * t.x;
+        w = __builtin_amdgcn_rcpf((L * L) + q * q);
+    } else {
+        u = cv.mag * yr;
+        w = rL * rL;
+    }
+    const float uidx = (u - a.u0) * a.inv_du;
+    int uf = (int)uidx;
+    uf = uf < 0 ? 0 : (uf > a.nu - 2 ? a.nu - 2 : uf);
+    cv.uf = (uint32_t)uf;
+    cv.ru = uidx - (float)uf;
+    cv.coef = w * t.w;  // t.w = sid * dbeta * div_factor
+    return cv;
+}
+
 // d = fl(fl(mag * z) - v[0]) of slice iz, the quantity of the exact v test.
 __device__ __forceinline__ float v_coord(const BpArgs &a, float mag, int iz, float halfz) {
     float z = ((float)iz - halfz) * a.vh + a.cz;
@@ -495,25 +527,46 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
             for (int q = 0; q < NQ; ++q)
                 if (srow[q] < w.h) dst[256 * q] = stage[q];
         };
+        // Per-view table entries pass through LDS in batches of 64 views:
+        // s_vt[i] = {cos, sin, dbeta, weight} of view j0 + i, s_win[i] = window
+        // of view j0 + i + 1.  (Read from global memory inside the loop they
+        // would be vector loads queued behind the window prefetch, and
+        // waiting for them would wait for the prefetch.)
+        __shared__ float4 s_vt[64];
+        __shared__ BpWin s_win[64];
+        auto load_tables = [&](int j0) {
+            if (tid < 64) {
+                if (j0 + tid < a.nviews) s_vt[tid] = a.vt[j0 + tid];
+            } else if (tid < 128) {
+                const int jj = j0 + 1 + (tid - 64);
+                if (jj < a.nviews) s_win[tid - 64] = wt[jj];
+            }
+        };
         float x, y;
         voxel_xy(a, ix, iy, x, y);
         const float halfz = (float)(a.nz - 1) / 2.0f;
         const float zfirst = ((float)z0 - halfz) * a.vh + a.cz;
         const float zstep_t = a.vh * a.vinv;
+        const uint32_t row_stride = (uint32_t)pitch * (uint32_t)sizeof(Raw);  // bytes per window row
 
+        load_tables(0);
         BpWin wc = wt[0];
         fetch(0, wc);
         put(0, wc);
         __syncthreads();
         for (int j = 0; j < a.nviews; ++j) {
+            if (j && (j & 63) == 0) {  // every wave has finished the previous batch's entries
+                load_tables(j);
+                __syncthreads();
+            }
             const bool more = j + 1 < a.nviews;
             BpWin wn = wc;
             if (more) {
-                wn = wt[j + 1];
+                wn = s_win[j & 63];
                 fetch(j + 1, wn);
             }
             if (alive) {
-                const ColumnView cv = column_view(a, x, y, a.vt[j]);
+                const FastView cv = column_view_fast(a, x, y, s_vt[j & 63]);
                 // row coordinate relative to the window's first row; the
                 // subtraction is exact when 0 <= r0 <= t0, which holds
                 // whenever a slice of the chunk passes the range tests
@@ -524,14 +577,24 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
                 lc = lc < 0 ? 0 : (lc > wmax ? wmax : lc);
                 // largest float below h: a clamped row coordinate floors to <= h - 1
                 const float tlim = wc.h > 0 ? __uint_as_float(__float_as_uint((float)wc.h) - 1u) : 0.f;
-                const Raw *src = lds + (j & 1) * cells + lc;
+                const unsigned char *src =
+                    smem + (uint32_t)((j & 1) * cells + lc) * (uint32_t)sizeof(Raw);
+                // reads in batches of 8, all in flight before the first is used
 #pragma unroll
-                for (int k = 0; k < kLdsZC; ++k) {
-                    float tk = fmaf((float)k, dt, t0);
-                    tk = __builtin_amdgcn_fmed3f(tk, 0.f, tlim);
-                    const uint32_t vf = (uint32_t)tk;
-                    const float rv = __builtin_amdgcn_fractf(tk);
-                    acc[k] = fmaf(cv.coef, bilerp(T::cvt(src[vf * (uint32_t)pitch]), cv.ru, rv), acc[k]);
+                for (int k0 = 0; k0 < kLdsZC; k0 += 8) {
+                    Raw q[8];
+                    float rv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        float tk = fmaf((float)(k0 + i), dt, t0);
+                        tk = __builtin_amdgcn_fmed3f(tk, 0.f, tlim);
+                        const uint32_t vf = (uint32_t)tk;
+                        rv[i] = __builtin_amdgcn_fractf(tk);
+                        q[i] = *reinterpret_cast<const Raw *>(src + __umul24(vf, row_stride));
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        acc[k0 + i] = fmaf(cv.coef, bilerp(T::cvt(q[i]), cv.ru, rv[i]), acc[k0 + i]);
                 }
             }
             if (more) put((j + 1) & 1, wn);
