@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--workload", default="fdk", choices=["fdk", "fbp2d", "fp3d", "ir"],
                     help="fdk: the bench line; the others print their own measurement")
     ap.add_argument("--fp-views", type=int, default=1080, help="fp3d: views projected")
+    ap.add_argument("--fp-kernel", default="auto", choices=["auto", "direct", "cells"],
+                    help="fp3d: forward projector")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB 5 times (profiler counter calibration)")
     args = ap.parse_args()

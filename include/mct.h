@@ -224,6 +224,25 @@ int mct_bp_last_kernel(const mct_ctx *ctx);
 int mct_forward_project(mct_plan *plan, const float *vol, float *out, const float *ray_mask,
                         int32_t oversample);
 
+/* Forward projector used by mct_forward_project:
+ *   MCT_FP_AUTO    the cell kernel where each order of cells is under 4 GiB,
+ *                  else the direct one;
+ *   MCT_FP_DIRECT  every sample gathers its 8 (cone) or 4 (fan) voxels from
+ *                  the volume as given;
+ *   MCT_FP_CELLS   the volume is first repacked, in context-owned scratch,
+ *                  into cells of bilinear coefficients (cone: planes z and
+ *                  z+1, 32 B; fan: 16 B) in an x-major and a y-major order;
+ *                  each view reads the order whose fast axis follows its
+ *                  detector rows, so a sample is one or two coalesced 16-byte
+ *                  loads (MCT_ERR_UNSUPPORTED where it does not fit).
+ * Both take the same samples with the same in-bounds tests; the cells hold
+ * FP64-rounded coefficients, so the sums agree to rounding. */
+typedef enum mct_fp_kernel { MCT_FP_AUTO = 0, MCT_FP_DIRECT = 1, MCT_FP_CELLS = 2 } mct_fp_kernel;
+int mct_set_fp_kernel(mct_ctx *ctx, int32_t kind);
+/* Kernel the last mct_forward_project call on ctx ran (MCT_FP_DIRECT or
+ * MCT_FP_CELLS; 0 none yet; -1 for a NULL ctx). */
+int mct_fp_last_kernel(const mct_ctx *ctx);
+
 /* Mgfbp_ir.GenEffectiveMapForwardProjectionAgent
  * run_mgfbp_iterative_recon_IRN_CGLS.py:679-833: 1 for rays whose samples
  * inside the in-plane FOV disc all stay within the volume's z range, else 0

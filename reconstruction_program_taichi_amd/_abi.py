@@ -22,6 +22,7 @@ MCT_PLAIN, MCT_QUAD, MCT_PAIR = 0, 1, 2
 MCT_FILTER_F32_DIRECT, MCT_FILTER_BF16_MFMA, MCT_FILTER_BF16X3_MFMA = 0, 1, 2
 
 MCT_BP_AUTO, MCT_BP_DIRECT, MCT_BP_LDS = 0, 1, 2
+MCT_FP_AUTO, MCT_FP_DIRECT, MCT_FP_CELLS = 0, 1, 2
 
 ABI_VERSION = 3
 
@@ -84,6 +85,8 @@ SIGNATURES = {
     "mct_set_bp_tile": (C.c_int, [_P, _I32, _I32]),
     "mct_set_bp_kernel": (C.c_int, [_P, _I32]),
     "mct_bp_last_kernel": (C.c_int, [_P]),
+    "mct_set_fp_kernel": (C.c_int, [_P, _I32]),
+    "mct_fp_last_kernel": (C.c_int, [_P]),
     "mct_backproject_batch": (C.c_int, [_P, C.POINTER(BpJob), _I32]),
     "mct_forward_project": (C.c_int, [_P, _P, _P, _P, _I32]),
     "mct_ray_mask": (C.c_int, [_P, _P]),

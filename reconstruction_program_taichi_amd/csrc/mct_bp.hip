@@ -34,9 +34,7 @@
 
 namespace {
 
-// Word 3 of a raw buffer resource on gfx9-family GPUs (gfx950 included):
-// 32-bit untyped data, no swizzle.
-constexpr int kBufferDword3 = 0x00020000;
+using mct::kBufferDword3;
 
 // Window of packed cells of one (tile, chunk, view) for the LDS-staged cone
 // kernel: first cell (column c0, row r0) and extent w x h; all inside

@@ -172,6 +172,7 @@ int mct_destroy(mct_ctx *ctx) {
     if (ctx->toep) hipFree(ctx->toep);
     if (ctx->taps_dev) hipFree(ctx->taps_dev);
     if (ctx->vkern_dev) hipFree(ctx->vkern_dev);
+    if (ctx->fp_pack) hipFree(ctx->fp_pack);
     if (ctx->batch_host) hipHostFree(ctx->batch_host);
     if (ctx->batch_dev) hipFree(ctx->batch_dev);
     if (ctx->batch_ev) hipEventDestroy(ctx->batch_ev);
@@ -211,6 +212,16 @@ int mct_set_bp_kernel(mct_ctx *ctx, int32_t kind) {
 }
 
 int mct_bp_last_kernel(const mct_ctx *ctx) { return ctx ? ctx->bp_last : -1; }
+
+int mct_set_fp_kernel(mct_ctx *ctx, int32_t kind) {
+    if (!ctx) return mct::set_error(MCT_ERR_INVALID, "ctx is NULL");
+    if (kind < MCT_FP_AUTO || kind > MCT_FP_CELLS)
+        return mct::set_error(MCT_ERR_INVALID, "forward-projector kind must be 0, 1 or 2 (got %d)", kind);
+    ctx->fp_kernel = kind;
+    return MCT_OK;
+}
+
+int mct_fp_last_kernel(const mct_ctx *ctx) { return ctx ? ctx->fp_last : -1; }
 
 int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
     if (!out) return mct::set_error(MCT_ERR_INVALID, "out is NULL");

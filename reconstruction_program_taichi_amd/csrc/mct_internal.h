@@ -16,6 +16,11 @@ struct mct_ctx {
     int bp_batch = 0;  // loads issued together by the cone back-projector (0: default)
     int bp_kernel = 0; // cone back-projector: 0 automatic, 1 direct gather, 2 LDS-staged
     int bp_last = 0;   // kernel of the last mct_backproject_fbp (mct_bp_last_kernel)
+    int fp_kernel = 0; // forward projector: 0 automatic, 1 direct taps, 2 packed cells
+    int fp_last = 0;   // kernel of the last mct_forward_project (mct_fp_last_kernel)
+    // packed volume of the forward projector (two orders of bilinear cells)
+    float *fp_pack = nullptr;
+    int64_t fp_pack_cap = 0;  // floats
     // partial sums of the deterministic FP64 reductions
     double *partials = nullptr;
     int64_t partials_cap = 0;
@@ -71,6 +76,10 @@ struct mct_plan {
 namespace mct {
 
 constexpr float kPI = 3.1415926536f;  // the reference's PI (run_mgfbp.py:16)
+
+// Word 3 of a raw buffer resource on gfx9-family GPUs (gfx950 included):
+// 32-bit untyped data, no swizzle.  Loads beyond num_records return 0.
+constexpr int kBufferDword3 = 0x00020000;
 
 // Records the message returned by mct_last_error() for the calling thread.
 int set_error(int code, const char *fmt, ...);

@@ -71,6 +71,19 @@ class Context:
         'lds' (detector windows staged in LDS)."""
         call("mct_set_bp_kernel", self.h, self.BP_KERNELS[kind])
 
+    FP_KERNELS = {"auto": _abi.MCT_FP_AUTO, "direct": _abi.MCT_FP_DIRECT,
+                  "cells": _abi.MCT_FP_CELLS}
+
+    def set_fp_kernel(self, kind: str) -> None:
+        """Forward projector: 'auto', 'direct' (voxel gathers from the volume
+        as given) or 'cells' (bilinear cells in two orders, in context
+        scratch)."""
+        call("mct_set_fp_kernel", self.h, self.FP_KERNELS[kind])
+
+    def fp_last_kernel(self) -> str:
+        k = _abi.lib().mct_fp_last_kernel(self.h)
+        return {0: "none", 1: "direct", 2: "cells"}.get(k, str(k))
+
     def bp_last_kernel(self) -> str:
         """Which back-projector the last FBP back-projection on this context ran."""
         k = _abi.lib().mct_bp_last_kernel(self.h)

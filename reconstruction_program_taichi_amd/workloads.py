@@ -86,6 +86,7 @@ def fp3d(args) -> dict:
     vol = phantom.voxelize(ell, 512, 256, 0.3, 0.3, clip_r=76.0, supersample=1, device=dev)
     out = torch.empty((256, nv, 1024), dtype=torch.float32, device=dev)
     plan = Plan(fpj.ctx(), spec)
+    fpj.ctx().set_fp_kernel(getattr(args, "fp_kernel", "auto"))
     plan.forward_project(vol, out)  # warm-up
     t = _sync_time(lambda: plan.forward_project(vol, out), max(1, min(args.steps, 3)))
     diag = float(np.sqrt(np.float32(2) * np.float32(0.3) ** 2 + np.float32(0.3) ** 2))
@@ -95,6 +96,7 @@ def fp3d(args) -> dict:
     rays = 256 * nv * 1024
     S = rays * steps
     return {"workload": f"cone forward projection 512x512x256 -> {nv} views x 256 x 1024",
+            "kernel": fpj.ctx().fp_last_kernel(),
             "ms": round(t * 1e3, 3), "rays": rays, "steps_per_ray": steps,
             "samples_per_s": round(S / t / 1e9, 2), "unit": "G samples/s (reference step count)"}
 

@@ -35,21 +35,56 @@ def _vol(spec, seed):
     return v
 
 
-@pytest.mark.parametrize("kind", ["fan", "cone", "cone_top", "fan_os2"])
-def test_forward_project(ctx, kind):
+def _fp_spec(kind):
     os_ = 2 if kind == "fan_os2" else 1
     if kind.startswith("fan"):
         spec = fan_case(nx=40, nv=24, nu=60 * os_, w=0.8 / os_, rows=3)
+    elif kind == "cone_offcentre":
+        spec = cone_case(nx=33, nz=15, nv=20, nu=48, nrows=24)
+        spec.cx, spec.cy = 1.3, -0.8
     else:
         spec = cone_case(nx=32, nz=16, nv=20, nu=48, nrows=24, top_row=(kind == "cone_top"))
     spec.fpj_step = 0.2
+    return spec, os_
+
+
+@pytest.mark.parametrize("kernel", ["cells", "direct"])
+@pytest.mark.parametrize("kind", ["fan", "cone", "cone_top", "fan_os2", "cone_offcentre"])
+def test_forward_project(ctx, kind, kernel):
+    spec, os_ = _fp_spec(kind)
     vol = _vol(spec, 1)
     plan = Plan(ctx, spec)
     out = torch.zeros((spec.n_v, spec.n_views, spec.n_u // os_), dtype=torch.float32, device=DEV)
-    plan.forward_project(dev(vol), out, None, os_)
+    ctx.set_fp_kernel(kernel)
+    try:
+        plan.forward_project(dev(vol), out, None, os_)
+        assert ctx.fp_last_kernel() == kernel
+    finally:
+        ctx.set_fp_kernel("auto")
     got = host(out)
     ref = recon.forward_project(vol, oracle_dict(spec), oversample=os_, dtype=np.float32)
     assert rel_l2(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["fan", "cone", "cone_offcentre", "fan_os2"])
+def test_forward_project_cells_equal_direct(ctx, kind):
+    """Same samples, in-bounds tests and summation order; the cells hold
+    FP64-rounded bilinear coefficients where the direct kernel interpolates
+    raw voxels in FP32: equal to rounding, and zero on the same rays."""
+    spec, os_ = _fp_spec(kind)
+    vol = dev(_vol(spec, 4))
+    plan = Plan(ctx, spec)
+    outs = {}
+    for kernel in ("cells", "direct"):
+        ctx.set_fp_kernel(kernel)
+        try:
+            o = torch.zeros((spec.n_v, spec.n_views, spec.n_u // os_), dtype=torch.float32, device=DEV)
+            plan.forward_project(vol, o, None, os_)
+            outs[kernel] = host(o)
+        finally:
+            ctx.set_fp_kernel("auto")
+    assert np.array_equal(outs["cells"] != 0, outs["direct"] != 0)
+    assert rel_l2(outs["cells"], outs["direct"]) <= 2e-6
 
 
 def test_forward_project_count_mode_v3(ctx):
