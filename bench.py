@@ -107,6 +107,8 @@ def main() -> int:
     ap.add_argument("--bp-batch", type=int, default=0, help="loads per batch (0: library default)")
     ap.add_argument("--bp-kernel", default="auto", choices=["auto", "direct", "lds"],
                     help="cone back-projector: LDS-staged windows or per-update gathers")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the back-projector candidates during warm-up and keep the fastest")
     ap.add_argument("--sinogram", default="phantom", choices=["phantom", "random"],
                     help="analytic ellipsoid projections, or seeded uniform noise (profiler runs)")
     ap.add_argument("--workload", default="fdk", choices=["fdk", "fbp2d", "fp3d", "ir"],
@@ -160,6 +162,7 @@ def main() -> int:
     if args.bp_tile or args.bp_batch:
         fbp.ctx().set_bp_tile(args.bp_tile, args.bp_batch)
     fbp.ctx().set_bp_kernel(args.bp_kernel)
+    fbp.autotune = args.autotune  # tunes on the first (warm-up) step, outside the timed region
     stream = torch.cuda.current_stream(dev)
 
     bp_times = []
@@ -205,7 +208,9 @@ def main() -> int:
                  if args.sinogram == "phantom" else "synthetic: seeded uniform random sinogram"),
         "config": {"workload": "cone FDK 512x512x256 volume, 1080 views, 1024x256 detector",
                    "sinogram_storage": args.storage, "ramp_filter": args.filter,
-                   "bp_ms": round(bp_ms, 3), "bp_gups": round(U / (bp_ms * 1e-3) / 1e9, 2)},
+                   "bp_ms": round(bp_ms, 3), "bp_gups": round(U / (bp_ms * 1e-3) / 1e9, 2),
+                   **({"autotune": {"choice": fbp.tuned, "candidates_ms": fbp.tuner.timings}}
+                      if fbp.tuned else {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((kernel, args.storage, args.bp_tile,

@@ -186,6 +186,19 @@ typedef struct mct_bp_job {
  * context are serialised on the host by that copy. */
 int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs);
 
+/* The same batched back-projection, prepared once and launched many times:
+ * mct_bp_batch_create validates the jobs, builds the plans' tables and copies
+ * the job table to device memory (synchronously); mct_bp_batch_launch then
+ * only launches the persistent kernel on the context's current stream - no
+ * copy, allocation or synchronisation - so it can be captured into a HIP
+ * graph and replayed.  Every replay back-projects the same sinogram buffers
+ * into the same volumes (whatever they hold at that time).  Plans and buffers
+ * must outlive the batch. */
+typedef struct mct_bp_batch mct_bp_batch;
+int mct_bp_batch_create(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs, mct_bp_batch **out);
+int mct_bp_batch_launch(mct_bp_batch *batch);
+int mct_bp_batch_destroy(mct_bp_batch *batch);
+
 /* Tile shape of the axial back-projector (autotuner hook): zc slices per
  * thread (4, 8, 16 or 32) and, for cone beam, batch taps loads issued
  * together (4, 8 or 16, at most zc); 0 restores a default.  Applies to the

@@ -88,6 +88,9 @@ SIGNATURES = {
     "mct_set_fp_kernel": (C.c_int, [_P, _I32]),
     "mct_fp_last_kernel": (C.c_int, [_P]),
     "mct_backproject_batch": (C.c_int, [_P, C.POINTER(BpJob), _I32]),
+    "mct_bp_batch_create": (C.c_int, [_P, C.POINTER(BpJob), _I32, C.POINTER(_P)]),
+    "mct_bp_batch_launch": (C.c_int, [_P]),
+    "mct_bp_batch_destroy": (C.c_int, [_P]),
     "mct_forward_project": (C.c_int, [_P, _P, _P, _P, _I32]),
     "mct_ray_mask": (C.c_int, [_P, _P]),
     "mct_add_noise": (C.c_int, [_P, _P, _I64, _F32, _U64]),

@@ -32,6 +32,8 @@
 // row coordinate and its two ends are confirmed with the exact test.
 #include "mct_internal.h"
 
+#include <vector>
+
 namespace {
 
 using mct::kBufferDword3;
@@ -1049,6 +1051,97 @@ int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
     return MCT_OK;
 }
 
+// Largest number of jobs in one batched launch.
+constexpr int kMaxBatchJobs = 4096;
+
+// Validates the jobs and fills the job table: args[i] and offs[i] (first
+// work item of job i; offs[njobs] = total).  Cone jobs get the LDS-staged
+// tile where the context allows it (building their plans' tables on first
+// use); lds_cells is the largest LDS window among them (0: none).
+int fill_batch(mct_ctx *ctx, const mct_bp_job *jobs, int njobs, int dtype, BpArgs *args, int *offs,
+               int &total_out, int &lds_cells) {
+    MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
+    int64_t total = 0;
+    lds_cells = 0;
+    for (int i = 0; i < njobs; ++i) {
+        const mct_bp_job &jb = jobs[i];
+        MCT_CHECK(jb.plan && jb.filt && jb.vol, "job %d: NULL plan, sinogram or volume", i);
+        MCT_CHECK(jb.plan->ctx == ctx, "job %d: plan belongs to another context", i);
+        MCT_CHECK(jb.dtype == dtype, "job %d: all jobs of a batch must share one dtype", i);
+        const mct_geometry &g = jb.plan->g;
+        MCT_CHECK(g.view_mode == 1 && !g.apply_pmatrix && !g.curved,
+                  "job %d: batched back-projection supports flat-detector axial jobs", i);
+        MCT_CHECK(jb.layout == (g.cone ? MCT_QUAD : MCT_PAIR), "job %d: cone reads QUAD, fan PAIR", i);
+        if (!g.cone) MCT_CHECK(g.n_v == g.img_dim_z, "job %d: fan beam needs one row per slice", i);
+        if (g.cone) MCT_CHECK(g.det_h > 0.f, "job %d: cone beam needs det_h > 0", i);
+        BpArgs a = make_args(jb.plan);
+        a.sino = jb.filt;
+        a.layout = jb.layout;
+        a.vol = jb.vol;
+        const int zc = g.cone ? 16 : 8;
+        a.tiles_x = (g.img_dim + 15) / 16;
+        a.tiles_y = a.tiles_x;
+        a.nzc = (g.img_dim_z + zc - 1) / zc;
+        if (g.cone && ctx->bp_kernel != MCT_BP_DIRECT) {
+            const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
+            int rc = prepare_cone_lds(jb.plan, a, dtype, automatic);
+            if (rc == MCT_OK) {
+                const int cells = a.pitch * a.rows;
+                lds_cells = cells > lds_cells ? cells : lds_cells;
+            } else if (rc != MCT_ERR_UNSUPPORTED) {
+                return rc;
+            } else if (!automatic) {
+                return mct::set_error(MCT_ERR_UNSUPPORTED,
+                                      "job %d: LDS-staged back-projection does not fit its geometry", i);
+            }
+        }
+        args[i] = a;
+        offs[i] = (int)total;
+        total += (int64_t)a.tiles_x * a.tiles_y * a.nzc;
+        MCT_CHECK(total < (int64_t)1 << 30, "batch too large");
+    }
+    offs[njobs] = (int)total;
+    total_out = (int)total;
+    return MCT_OK;
+}
+
+// The persistent batch kernel over a device job table, on the context's
+// stream: as many workgroups as stay resident together.  Launches only (no
+// copy, allocation or synchronisation), so it can be captured in a graph.
+int launch_batch(mct_ctx *ctx, const BpArgs *dargs, const int *doffs, int njobs, int total, int dtype,
+                 int lds_cells) {
+    const size_t shmem = lds_cells ? lds_bytes(lds_cells, dtype) : 0;
+    const int nq = lds_cells ? lds_slots(lds_cells) : 0;
+#define BATCH_KERNEL(DT, NQ) bp_batch_kernel<DT, NQ>
+#define BATCH_LAUNCH(DT, NQ)                                                                        \
+    do {                                                                                            \
+        int per_cu = 0;                                                                             \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BATCH_KERNEL(DT, NQ), 256, shmem) \
+                != hipSuccess || per_cu < 1)                                                        \
+            per_cu = 1;                                                                             \
+        int64_t grid = (int64_t)ctx->cu_count * per_cu;                                             \
+        grid = grid < total ? grid : total;                                                         \
+        hipLaunchKernelGGL(BATCH_KERNEL(DT, NQ), dim3((unsigned)grid), dim3(256), shmem,            \
+                           ctx->stream, dargs, doffs, njobs, total);                                \
+    } while (0)
+#define BATCH_DT(NQ)                                         \
+    if (dtype == MCT_F32) BATCH_LAUNCH(MCT_F32, NQ);         \
+    else if (dtype == MCT_F16) BATCH_LAUNCH(MCT_F16, NQ);    \
+    else BATCH_LAUNCH(MCT_BF16, NQ);
+    switch (nq) {
+        case 0: BATCH_DT(0) break;
+        case 2: BATCH_DT(2) break;
+        case 4: BATCH_DT(4) break;
+        case 6: BATCH_DT(6) break;
+        default: BATCH_DT(8) break;
+    }
+#undef BATCH_DT
+#undef BATCH_LAUNCH
+#undef BATCH_KERNEL
+    MCT_LAUNCHED();
+    return MCT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1171,10 +1264,10 @@ int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, 
 
 int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     MCT_CHECK(ctx && jobs, "NULL argument");
-    MCT_CHECK(njobs >= 0 && njobs <= 4096, "njobs must be in [0, 4096] (got %d)", njobs);
+    MCT_CHECK(njobs >= 0 && njobs <= kMaxBatchJobs, "njobs must be in [0, %d] (got %d)", kMaxBatchJobs,
+              njobs);
     if (njobs == 0) return MCT_OK;
     const int dtype = jobs[0].dtype;
-    MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
     const size_t args_bytes = sizeof(BpArgs) * (size_t)njobs;
     const size_t bytes = args_bytes + sizeof(int) * ((size_t)njobs + 1);
     if (ctx->batch_cap < (int64_t)bytes) {
@@ -1195,80 +1288,64 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     MCT_HIP(hipEventSynchronize(ctx->batch_ev));
     BpArgs *args = reinterpret_cast<BpArgs *>(ctx->batch_host);
     int *offs = reinterpret_cast<int *>(reinterpret_cast<char *>(ctx->batch_host) + args_bytes);
-    int64_t total = 0;
-    int lds_cells = 0;  // largest LDS window among the staged cone jobs
-    for (int i = 0; i < njobs; ++i) {
-        const mct_bp_job &jb = jobs[i];
-        MCT_CHECK(jb.plan && jb.filt && jb.vol, "job %d: NULL plan, sinogram or volume", i);
-        MCT_CHECK(jb.plan->ctx == ctx, "job %d: plan belongs to another context", i);
-        MCT_CHECK(jb.dtype == dtype, "job %d: all jobs of a batch must share one dtype", i);
-        const mct_geometry &g = jb.plan->g;
-        MCT_CHECK(g.view_mode == 1 && !g.apply_pmatrix && !g.curved,
-                  "job %d: batched back-projection supports flat-detector axial jobs", i);
-        MCT_CHECK(jb.layout == (g.cone ? MCT_QUAD : MCT_PAIR), "job %d: cone reads QUAD, fan PAIR", i);
-        if (!g.cone) MCT_CHECK(g.n_v == g.img_dim_z, "job %d: fan beam needs one row per slice", i);
-        if (g.cone) MCT_CHECK(g.det_h > 0.f, "job %d: cone beam needs det_h > 0", i);
-        BpArgs a = make_args(jb.plan);
-        a.sino = jb.filt;
-        a.layout = jb.layout;
-        a.vol = jb.vol;
-        const int zc = g.cone ? 16 : 8;
-        a.tiles_x = (g.img_dim + 15) / 16;
-        a.tiles_y = a.tiles_x;
-        a.nzc = (g.img_dim_z + zc - 1) / zc;
-        if (g.cone && ctx->bp_kernel != MCT_BP_DIRECT) {
-            const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
-            int rc = prepare_cone_lds(jb.plan, a, dtype, automatic);
-            if (rc == MCT_OK) {
-                const int cells = a.pitch * a.rows;
-                lds_cells = cells > lds_cells ? cells : lds_cells;
-            } else if (rc != MCT_ERR_UNSUPPORTED) {
-                return rc;
-            } else if (!automatic) {
-                return mct::set_error(MCT_ERR_UNSUPPORTED,
-                                      "job %d: LDS-staged back-projection does not fit its geometry", i);
-            }
-        }
-        args[i] = a;
-        offs[i] = (int)total;
-        total += (int64_t)a.tiles_x * a.tiles_y * a.nzc;
-        MCT_CHECK(total < (int64_t)1 << 30, "batch too large");
-    }
-    offs[njobs] = (int)total;
+    int total = 0, lds_cells = 0;
+    int rc = fill_batch(ctx, jobs, njobs, dtype, args, offs, total, lds_cells);
+    if (rc) return rc;
     MCT_HIP(hipMemcpyAsync(ctx->batch_dev, ctx->batch_host, bytes, hipMemcpyHostToDevice, ctx->stream));
     MCT_HIP(hipEventRecord(ctx->batch_ev, ctx->stream));
     const BpArgs *dargs = reinterpret_cast<const BpArgs *>(ctx->batch_dev);
     const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(ctx->batch_dev) + args_bytes);
-    const size_t shmem = lds_cells ? lds_bytes(lds_cells, dtype) : 0;
-    const int nq = lds_cells ? lds_slots(lds_cells) : 0;
-    // a persistent grid: as many workgroups as stay resident together
-#define BATCH_KERNEL(DT, NQ) bp_batch_kernel<DT, NQ>
-#define BATCH_LAUNCH(DT, NQ)                                                                        \
-    do {                                                                                            \
-        int per_cu = 0;                                                                             \
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, BATCH_KERNEL(DT, NQ), 256, shmem) \
-                != hipSuccess || per_cu < 1)                                                        \
-            per_cu = 1;                                                                             \
-        int64_t grid = (int64_t)ctx->cu_count * per_cu;                                             \
-        grid = grid < total ? grid : total;                                                         \
-        hipLaunchKernelGGL(BATCH_KERNEL(DT, NQ), dim3((unsigned)grid), dim3(256), shmem,            \
-                           ctx->stream, dargs, doffs, njobs, (int)total);                           \
-    } while (0)
-#define BATCH_DT(NQ)                                         \
-    if (dtype == MCT_F32) BATCH_LAUNCH(MCT_F32, NQ);         \
-    else if (dtype == MCT_F16) BATCH_LAUNCH(MCT_F16, NQ);    \
-    else BATCH_LAUNCH(MCT_BF16, NQ);
-    switch (nq) {
-        case 0: BATCH_DT(0) break;
-        case 2: BATCH_DT(2) break;
-        case 4: BATCH_DT(4) break;
-        case 6: BATCH_DT(6) break;
-        default: BATCH_DT(8) break;
+    return launch_batch(ctx, dargs, doffs, njobs, total, dtype, lds_cells);
+}
+
+int mct_bp_batch_create(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs, mct_bp_batch **out) {
+    MCT_CHECK(out, "out is NULL");
+    *out = nullptr;
+    MCT_CHECK(ctx && jobs, "NULL argument");
+    MCT_CHECK(njobs >= 1 && njobs <= kMaxBatchJobs, "njobs must be in [1, %d] (got %d)", kMaxBatchJobs,
+              njobs);
+    const int dtype = jobs[0].dtype;
+    std::vector<BpArgs> args((size_t)njobs);
+    std::vector<int> offs((size_t)njobs + 1);
+    int total = 0, lds_cells = 0;
+    int rc = fill_batch(ctx, jobs, njobs, dtype, args.data(), offs.data(), total, lds_cells);
+    if (rc) return rc;
+    const size_t args_bytes = sizeof(BpArgs) * (size_t)njobs;
+    void *dev = nullptr;
+    MCT_HIP(hipMalloc(&dev, args_bytes + sizeof(int) * offs.size()));
+    hipError_t e = hipMemcpy(dev, args.data(), args_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(reinterpret_cast<char *>(dev) + args_bytes, offs.data(), sizeof(int) * offs.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(dev);
+        return mct::set_error(MCT_ERR_DEVICE, "batch table upload: %s", hipGetErrorString(e));
     }
-#undef BATCH_DT
-#undef BATCH_LAUNCH
-#undef BATCH_KERNEL
-    MCT_LAUNCHED();
+    mct_bp_batch *b = new mct_bp_batch();
+    b->ctx = ctx;
+    b->device = ctx->device;
+    b->dev = dev;
+    b->args_bytes = args_bytes;
+    b->njobs = njobs;
+    b->total = total;
+    b->dtype = dtype;
+    b->lds_cells = lds_cells;
+    *out = b;
+    return MCT_OK;
+}
+
+int mct_bp_batch_launch(mct_bp_batch *b) {
+    MCT_CHECK(b, "NULL batch");
+    const BpArgs *dargs = reinterpret_cast<const BpArgs *>(b->dev);
+    const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(b->dev) + b->args_bytes);
+    return launch_batch(b->ctx, dargs, doffs, b->njobs, b->total, b->dtype, b->lds_cells);
+}
+
+int mct_bp_batch_destroy(mct_bp_batch *b) {
+    if (!b) return MCT_OK;
+    hipSetDevice(b->device);
+    if (b->dev) hipFree(b->dev);
+    delete b;
     return MCT_OK;
 }
 

@@ -118,6 +118,18 @@ int ensure_buffer(mct_ctx *ctx, float **buf, int64_t *cap, int64_t n) {
     return MCT_OK;
 }
 
+int upload_cached(mct_ctx *ctx, float **buf, int64_t *cap, uint64_t *hash, const float *host,
+                  int64_t n) {
+    const uint64_t h = fnv1a(host, sizeof(float) * (size_t)n, (uint64_t)n);
+    if (*buf && *cap >= n && *hash == h) return MCT_OK;
+    int rc = ensure_buffer(ctx, buf, cap, n);
+    if (rc) return rc;
+    *hash = 0;  // invalid until the copy is queued
+    MCT_HIP(hipMemcpyAsync(*buf, host, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    *hash = h;
+    return MCT_OK;
+}
+
 float div_factor_of(const mct_geometry &g) {
     // run_mgfbp.py:1072-1084, FP32 as in the Taichi kernel.
     const float PI = kPI;

@@ -92,10 +92,9 @@ int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, i
     MCT_CHECK(in != out, "the row filter cannot run in place");
     if (method != MCT_FILTER_F32_DIRECT)
         return mct::filter_rows_mfma(ctx, in, out, n_rows, n, taps, scale, method);
-    int rc = mct::ensure_buffer(ctx, &ctx->taps_dev, &ctx->taps_cap, 2 * (int64_t)n);
+    int rc = mct::upload_cached(ctx, &ctx->taps_dev, &ctx->taps_cap, &ctx->taps_hash, taps,
+                                2 * (int64_t)n - 1);
     if (rc) return rc;
-    MCT_HIP(hipMemcpyAsync(ctx->taps_dev, taps, sizeof(float) * (2 * (size_t)n - 1),
-                           hipMemcpyHostToDevice, ctx->stream));
     size_t lds = sizeof(float) * (4 * (size_t)n + 2 * (size_t)n);
     int threads = (n + KB - 1) / KB;
     threads = ((threads + 63) / 64) * 64;

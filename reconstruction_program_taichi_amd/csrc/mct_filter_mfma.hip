@@ -166,11 +166,7 @@ float bf16_value(uint16_t h) {
     return f;
 }
 
-uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603ull) {
-    const unsigned char *b = (const unsigned char *)p;
-    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
-    return h;
-}
+using mct::fnv1a;
 
 }  // namespace
 

@@ -32,8 +32,10 @@ struct mct_ctx {
     uint64_t toep_hash = 0;
     float *taps_dev = nullptr;  // device copy of the ramp taps (direct method)
     int64_t taps_cap = 0;
+    uint64_t taps_hash = 0;
     float *vkern_dev = nullptr;  // device copy of the vertical Gaussian taps
     int64_t vkern_cap = 0;
+    uint64_t vkern_hash = 0;
     // job table of mct_backproject_batch: pinned staging, device copy, and the
     // event that marks the staging buffer free again
     void *batch_host = nullptr, *batch_dev = nullptr;
@@ -73,6 +75,16 @@ struct mct_plan {
     int bp_win_state = 0;            // 0 not built, 1 usable, -1 unusable geometry
 };
 
+// A prepared batched back-projection (mct_bp_batch_create): the job table in
+// device memory and the launch shape.
+struct mct_bp_batch {
+    mct_ctx *ctx = nullptr;
+    int device = 0;
+    void *dev = nullptr;  // BpArgs[njobs] followed by int offsets[njobs + 1]
+    size_t args_bytes = 0;
+    int njobs = 0, total = 0, dtype = 0, lds_cells = 0;
+};
+
 namespace mct {
 
 constexpr float kPI = 3.1415926536f;  // the reference's PI (run_mgfbp.py:16)
@@ -106,6 +118,20 @@ int set_error(int code, const char *fmt, ...);
     } while (0)
 
 int ensure_partials(mct_ctx *ctx, int64_t n);
+
+// FNV-1a hash of a byte range (tap-set identity for the upload caches).
+inline uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603ull) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// Copies n host floats to a context-owned device buffer unless the buffer
+// already holds exactly these values (same hash and count): a repeated call
+// then queues no copy, so the operation that uses the buffer only launches
+// kernels and can be captured into a graph once it has run.
+int upload_cached(mct_ctx *ctx, float **buf, int64_t *cap, uint64_t *hash, const float *host,
+                  int64_t n);
 
 // Grows a context-owned FP32 device buffer to at least n elements (waits for
 // the stream before freeing the old one).

@@ -182,10 +182,9 @@ int mct_filter_vertical(mct_ctx *ctx, const float *in, float *out, int32_t rows,
     MCT_CHECK(ctx && in && out && kern, "NULL argument");
     MCT_CHECK(in != out, "vertical filter cannot run in place");
     MCT_CHECK(rows > 0 && views > 0 && n_u > 0, "empty sinogram");
-    int rc = mct::ensure_buffer(ctx, &ctx->vkern_dev, &ctx->vkern_cap, 2 * (int64_t)rows - 1);
+    int rc = mct::upload_cached(ctx, &ctx->vkern_dev, &ctx->vkern_cap, &ctx->vkern_hash, kern,
+                                2 * (int64_t)rows - 1);
     if (rc) return rc;
-    MCT_HIP(hipMemcpyAsync(ctx->vkern_dev, kern, sizeof(float) * (2 * (size_t)rows - 1),
-                           hipMemcpyHostToDevice, ctx->stream));
     int64_t n = (int64_t)rows * views * n_u;
     hipLaunchKernelGGL(vertical_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream,
                        in, out, rows, views, n_u, ctx->vkern_dev, det_h);

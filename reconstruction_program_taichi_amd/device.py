@@ -126,19 +126,39 @@ class Context:
         if nbytes <= 0:
             raise ValueError(f"cannot pack shape {tuple(inp.shape)} as {layout}")
         out = torch.empty(int(nbytes), dtype=torch.uint8, device=inp.device)
+        return self.pack_into(inp, dtype, layout, out)
+
+    def pack_into(self, inp: torch.Tensor, dtype: str, layout: str, out: torch.Tensor) -> torch.Tensor:
+        """pack() into a preallocated byte buffer (no allocation: capturable)."""
+        rows, views, nu = inp.shape
+        nbytes = _abi.lib().mct_packed_bytes(rows, views, nu, DTYPES[dtype], LAYOUTS[layout])
+        if nbytes <= 0 or out.numel() * out.element_size() < nbytes or not out.is_contiguous():
+            raise ValueError(f"output buffer too small for packing {tuple(inp.shape)} as {layout}")
         call("mct_pack_sinogram", self.h, _f32(inp), rows, views, nu, _ptr(out), DTYPES[dtype],
              LAYOUTS[layout])
         return out
 
-    def backproject_batch(self, jobs) -> None:
-        """jobs: iterable of (plan, packed filtered sinogram, dtype, layout, vol)."""
-        jobs = list(jobs)
+    def _job_array(self, jobs):
         arr = (_abi.BpJob * max(len(jobs), 1))()
         for i, (plan, filt, dtype, layout, vol) in enumerate(jobs):
             if plan.ctx is not self:
                 raise ValueError("plan belongs to another context")
             arr[i] = _abi.BpJob(plan.h.value, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
-        call("mct_backproject_batch", self.h, arr, len(jobs))
+        return arr
+
+    def backproject_batch(self, jobs) -> None:
+        """jobs: iterable of (plan, packed filtered sinogram, dtype, layout, vol)."""
+        jobs = list(jobs)
+        call("mct_backproject_batch", self.h, self._job_array(jobs), len(jobs))
+
+    def prepare_batch(self, jobs) -> "BpBatch":
+        """A batched back-projection prepared once (job table on the device)
+        and launched many times; BpBatch.launch() only launches the kernel, so
+        it can be captured into a graph."""
+        jobs = list(jobs)
+        h = C.c_void_p()
+        call("mct_bp_batch_create", self.h, self._job_array(jobs), len(jobs), C.byref(h))
+        return BpBatch(self, h, jobs)
 
     def add_noise(self, sgm: torch.Tensor, photons: float, seed: int) -> torch.Tensor:
         call("mct_add_noise", self.h, _f32(sgm), sgm.numel(), float(photons), int(seed) & (2**64 - 1))
@@ -179,6 +199,29 @@ class Context:
 
     def cg_direction(self, d, r, s):
         call("mct_cg_direction", self.h, _f32(d), _f32(r), _ptr(s), d.numel())
+
+
+class BpBatch:
+    """A prepared batched back-projection (mct_bp_batch_*).  Keeps the jobs'
+    plans and tensors alive as long as it exists."""
+
+    def __init__(self, ctx: Context, h: C.c_void_p, jobs: list):
+        self.ctx, self.h, self._jobs = ctx, h, jobs
+
+    def launch(self) -> None:
+        call("mct_bp_batch_launch", self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _abi.lib().mct_bp_batch_destroy(self.h)
+            self.h = None
+            self._jobs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:

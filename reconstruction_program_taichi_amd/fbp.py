@@ -13,6 +13,9 @@ Additive config keys (no reference counterpart):
                                  back-projector reads (default f32)
   "RampFilterMethod": "f32"|"bf16"|"bf16x3"
                                  row-filter arithmetic (default f32)
+  "Autotune": true|false         time the back-projector's candidate kernels
+                                 and tiles on the first sinogram of a geometry
+                                 and keep the fastest (default false)
 """
 from __future__ import annotations
 
@@ -72,6 +75,11 @@ class Mgfbp:
         self.filter_method = cfg.get("RampFilterMethod", "f32")
         if self.filter_method not in ("f32", "bf16", "bf16x3"):
             fail("ERROR: RampFilterMethod can only be f32, bf16 or bf16x3!")
+        self.autotune = cfg.get("Autotune", False)
+        if not isinstance(self.autotune, bool):
+            fail("ERROR: Autotune must be true or false!")
+        self.tuner = None
+        self.tuned = None
         self.img_recon = np.zeros((self.img_dim_z, self.img_dim, self.img_dim), dtype=np.float32)
         self._device = device
         self._ctx = ctx
@@ -428,6 +436,12 @@ class Mgfbp:
         if out is None:
             out = torch.empty((self.img_dim_z, self.img_dim, self.img_dim), dtype=torch.float32,
                               device=sgm.device)
+        if (self.autotune and self.tuned is None and self.recon_view_mode == 1
+                and not self.bool_apply_pmatrix):
+            from .autotune import Autotuner
+
+            self.tuner = self.tuner or Autotuner(ctx)
+            self.tuned = self.tuner.tune_backprojector(plan, packed, self.storage, layout, out)
         plan.backproject_fbp(packed, self.storage, layout, out)
         return out
 
