@@ -341,17 +341,18 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
 }
 
 // fp_kernel on cells; xm / ym: the two record orders, `bytes` each (under
-// 4 GiB).  1-D grid: column blocks of 64 fastest, then views, then bands of 4
-// detector rows.
+// 4 GiB).  One wavefront traces 64 neighbouring columns of one detector row
+// and view; wavefronts are numbered column block fastest, then view, then
+// row (nwaves in all), four to a workgroup, so the views in flight share
+// detector rows and a single-row (fan) problem still fills every wavefront.
 template <bool CONE>
 __global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *xm, const float4 *ym,
-                                                       uint32_t bytes, int nub) {
-    const int b = blockIdx.x;
-    const int ub = b % nub, t = b / nub;
-    const int j = t % a.nviews, rb = t / a.nviews;
+                                                       uint32_t bytes, int nub, int nwaves) {
+    const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (w >= nwaves) return;  // a whole wavefront
+    const int ub = w % nub, t = w / nub;
+    const int j = t % a.nviews, vi = t / a.nviews;
     const int uo = ub * 64 + (threadIdx.x & 63);
-    const int vi = rb * 4 + (threadIdx.x >> 6);
-    if (vi >= a.nrows) return;  // a whole wavefront (a wavefront is one row)
     const bool inside = uo < a.nu_out;
     const size_t oidx = ((size_t)vi * a.nviews + j) * a.nu_out + (inside ? uo : 0);
     const bool masked = inside && a.mask && a.mask[oidx] != 1.0f;
@@ -564,11 +565,12 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         const int64_t nc = nx - 1;
         const int64_t planes = g.cone ? (int64_t)g.img_dim_z - 1 : (int64_t)g.img_dim_z;
         const int64_t bytes = nc * nc * planes * (g.cone ? 32 : 16);
-        const int64_t nub = (a.nu_out + 63) / 64, nrb = (a.nrows + 3) / 4;
-        const int64_t nblocks = nub * a.nviews * nrb;
-        // 32-bit record offsets; gridDim.z of the pack launch; 1-D grid size
+        const int64_t nub = (a.nu_out + 63) / 64;
+        const int64_t nwaves = nub * a.nviews * a.nrows;
+        const int64_t nblocks = (nwaves + 3) / 4;
+        // 32-bit record offsets; gridDim.z of the pack launch; wavefront index
         const bool fits = nc >= 1 && planes >= 1 && bytes < ((int64_t)1 << 32) && planes <= 65535 &&
-                          nblocks < ((int64_t)1 << 31);
+                          nwaves < ((int64_t)1 << 31);
         if (fits) {
             int rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, 2 * (bytes / 4));
             if (rc) return rc;
@@ -579,12 +581,12 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
                 hipLaunchKernelGGL(fp_pack_kernel<true>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub);
+                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves);
             } else {
                 hipLaunchKernelGGL(fp_pack_kernel<false>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub);
+                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves);
             }
             MCT_LAUNCHED();
             ctx->fp_last = MCT_FP_CELLS;

@@ -11,7 +11,8 @@ iterate state on the device and overlaps the work of different requests:
   * every scheduling step collects the FBP/FDK jobs whose sinogram has
     arrived, weights / filters / packs each, and back-projects all of them in
     ONE persistent launch (mct_backproject_batch), on compute stream A;
-  * CGLS-TV jobs run their iterations on compute stream B, overlapping A;
+  * CGLS-TV jobs run their iterations on a small pool of compute streams
+    (round robin), overlapping A and each other;
   * results go device -> host on the copy stream; buffers return to the
     budget when the copy has finished.
 
@@ -154,14 +155,25 @@ def run_plain(jobs: list[Job], device: int = 0) -> dict:
     reference's per-call pattern).  Returns latency/throughput statistics."""
     ctx = Context(device)
     dev = torch.device("cuda", device)
+    phases = {"setup": 0.0, "h2d": 0.0, "compute": 0.0, "d2h": 0.0}
+
+    def mark(name, t):
+        torch.cuda.synchronize(dev)
+        now = time.perf_counter()
+        phases[name] += now - t
+        return now
+
     t0 = time.perf_counter()
     for job in sorted(jobs, key=lambda j: j.arrival):
         now = time.perf_counter() - t0
         if now < job.arrival:
             time.sleep(job.arrival - now)
         job.t_admit = time.perf_counter() - t0
+        t = time.perf_counter()
         plan = Plan(ctx, job.spec)
+        t = mark("setup", t)
         sgm = job.sgm_host.to(dev)
+        t = mark("h2d", t)
         if job.kind == "cgls":
             x = torch.empty(job.vol_shape(), dtype=torch.float32, device=dev)
             cgls_run(ctx, plan, job, sgm, x)
@@ -169,29 +181,37 @@ def run_plain(jobs: list[Job], device: int = 0) -> dict:
             packed, layout = fbp_stages(ctx, plan, job, sgm)
             x = torch.empty(job.vol_shape(), dtype=torch.float32, device=dev)
             plan.backproject_fbp(packed, job.storage, layout, x)
+        t = mark("compute", t)
         job.result = x.cpu()
-        torch.cuda.synchronize(dev)
+        mark("d2h", t)
         plan.close()
         job.t_done = time.perf_counter() - t0
     ctx.close()
-    return stats(jobs)
+    out = stats(jobs)
+    out["phase_s"] = {k: round(v, 4) for k, v in phases.items()}
+    return out
 
 
 class Server:
     """Continuous-batching server on one device (see the module comment)."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, max_batch: int = 64):
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, max_batch: int = 64,
+                 cgls_streams: int = 3):
         self.device = device
         self.dev = torch.device("cuda", device)
         self.s_copy = torch.cuda.Stream(self.dev)
         self.s_a = torch.cuda.Stream(self.dev)
-        self.s_b = torch.cuda.Stream(self.dev)
         self.ctx_a = Context(device)
         self.ctx_a.bind_stream(self.s_a)
-        self.ctx_b = Context(device)
-        self.ctx_b.bind_stream(self.s_b)
         self.plans_a = PlanCache(self.ctx_a)
-        self.plans_b = PlanCache(self.ctx_b)
+        # CGLS jobs run on a small pool of streams (one context and plan cache
+        # each): a single-slice job's kernels do not fill the device alone
+        self.s_b = [torch.cuda.Stream(self.dev) for _ in range(max(1, cgls_streams))]
+        self.ctx_b = [Context(device) for _ in self.s_b]
+        for c, s in zip(self.ctx_b, self.s_b):
+            c.bind_stream(s)
+        self.plans_b = [PlanCache(c) for c in self.ctx_b]
+        self._next_b = 0
         self.budget = int(hbm_budget)
         self.used = 0
         self.max_batch = max_batch
@@ -207,11 +227,14 @@ class Server:
             return False
         self.used += need
         job.t_admit = now
+        if job.kind == "cgls":
+            job._dev["lane"] = self._next_b
+            self._next_b = (self._next_b + 1) % len(self.s_b)
         with torch.cuda.stream(self.s_copy):
             sgm = job.sgm_host.to(self.dev, non_blocking=True)
             # used on a compute stream later: keep the allocator from recycling
             # the block before that use has finished
-            sgm.record_stream(self.s_b if job.kind == "cgls" else self.s_a)
+            sgm.record_stream(self.s_b[job._dev["lane"]] if job.kind == "cgls" else self.s_a)
             job._dev["sgm"] = sgm
             ev = torch.cuda.Event()
             ev.record(self.s_copy)
@@ -238,12 +261,13 @@ class Server:
         self._copy_back(batch, done)
 
     def _launch_cgls(self, job: Job):
-        s = self.s_b
+        lane = job._dev["lane"]
+        s = self.s_b[lane]
         with torch.cuda.stream(s):
             s.wait_event(job._events["loaded"])
-            plan = self.plans_b.get(job.spec)
+            plan = self.plans_b[lane].get(job.spec)
             x = torch.empty(job.vol_shape(), dtype=torch.float32, device=self.dev)
-            cgls_run(self.ctx_b, plan, job, job._dev["sgm"], x)
+            cgls_run(self.ctx_b[lane], plan, job, job._dev["sgm"], x)
             job._dev["vol"] = x
             done = torch.cuda.Event()
             done.record(s)
@@ -304,14 +328,15 @@ class Server:
         return out
 
     def close(self):
-        for s in (self.s_copy, self.s_a, self.s_b):
+        for s in (self.s_copy, self.s_a, *self.s_b):
             s.synchronize()
-        for pc in (self.plans_a, self.plans_b):
+        for pc in (self.plans_a, *self.plans_b):
             for p in pc.plans.values():
                 p.close()
             pc.plans.clear()
         self.ctx_a.close()
-        self.ctx_b.close()
+        for c in self.ctx_b:
+            c.close()
 
 
 def stats(jobs: list[Job]) -> dict:
