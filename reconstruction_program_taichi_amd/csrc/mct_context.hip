@@ -185,6 +185,7 @@ int mct_destroy(mct_ctx *ctx) {
     if (ctx->taps_dev) hipFree(ctx->taps_dev);
     if (ctx->vkern_dev) hipFree(ctx->vkern_dev);
     if (ctx->fp_pack) hipFree(ctx->fp_pack);
+    if (ctx->fp_partial) hipFree(ctx->fp_partial);
     if (ctx->batch_host) hipHostFree(ctx->batch_host);
     if (ctx->batch_dev) hipFree(ctx->batch_dev);
     if (ctx->batch_ev) hipEventDestroy(ctx->batch_ev);

@@ -270,7 +270,8 @@ __device__ __forceinline__ int wave_max(int v) {
 // false: a lane without a ray).  Samples go four at a time, loads first.
 template <bool CONE>
 __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j,
-                             __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy, uint32_t pz) {
+                             __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy, uint32_t pz,
+                             int seg, int nseg) {
     float sz, ux, uy, uz;
     ray_setup(a, active ? ui : 0, vi, j, sz, ux, uy, uz);
     const float2 cs = a.ft[j];
@@ -284,6 +285,12 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
     }
     k_lo = __builtin_amdgcn_readfirstlane(wave_min(k_lo));
     k_hi = __builtin_amdgcn_readfirstlane(wave_max(k_hi));
+    if (nseg > 1 && k_hi > k_lo) {  // this wavefront's share of the steps
+        const int64_t len = (int64_t)k_hi - k_lo;
+        const int s0 = k_lo + (int)(len * seg / nseg), s1 = k_lo + (int)(len * (seg + 1) / nseg);
+        k_lo = s0;
+        k_hi = s1;
+    }
     const float ax = a.sid * c, ay = a.sid * s;
     const float bx = ux * c - uy * s, by = uy * c + ux * s;
     const float dl = a.step * a.diag;
@@ -347,9 +354,11 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
 // detector rows and a single-row (fan) problem still fills every wavefront.
 template <bool CONE>
 __global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *xm, const float4 *ym,
-                                                       uint32_t bytes, int nub, int nwaves) {
-    const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    if (w >= nwaves) return;  // a whole wavefront
+                                                       uint32_t bytes, int nub, int nwaves, int nseg,
+                                                       float *partial) {
+    const int wg = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (wg >= nwaves * nseg) return;  // a whole wavefront
+    const int seg = wg / nwaves, w = wg - seg * nwaves;
     const int ub = w % nub, t = w / nub;
     const int j = t % a.nviews, vi = t / a.nviews;
     const int uo = ub * 64 + (threadIdx.x & 63);
@@ -368,8 +377,23 @@ __global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *x
         __builtin_amdgcn_make_buffer_rsrc((void *)(ymaj ? ym : xm), 0, (int)bytes, kBufferDword3);
     float acc = 0.f;
     for (int i = 0; i < a.os; ++i)
-        acc += trace_cells<CONE>(a, active, uo * a.os + i, vi, j, rsrc, sx, sy, pz);
-    if (inside) a.out[oidx] = masked ? 0.f : acc / (float)a.os;
+        acc += trace_cells<CONE>(a, active, uo * a.os + i, vi, j, rsrc, sx, sy, pz, seg, nseg);
+    if (!inside) return;
+    const float v = masked ? 0.f : acc / (float)a.os;
+    if (nseg == 1)
+        a.out[oidx] = v;
+    else  // partial sum of segment seg; fp_segments_kernel adds them
+        partial[(size_t)seg * a.nrows * a.nviews * a.nu_out + oidx] = v;
+}
+
+// out[i] = sum of the nseg partial sums of ray i, in segment order.
+__global__ void fp_segments_kernel(const float *__restrict__ partial, int64_t n, int nseg,
+                                   float *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = partial[i];
+    for (int k = 1; k < nseg; ++k) s += partial[(size_t)k * n + i];
+    out[i] = s;
 }
 
 // Mgfbp_ir.GenEffectiveMapForwardProjectionAgent (cone beam): 0 when a
@@ -567,13 +591,25 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         const int64_t bytes = nc * nc * planes * (g.cone ? 32 : 16);
         const int64_t nub = (a.nu_out + 63) / 64;
         const int64_t nwaves = nub * a.nviews * a.nrows;
-        const int64_t nblocks = (nwaves + 3) / 4;
+        // Small problems (e.g. one fan slice) would not fill the device with
+        // one wavefront per 64 rays: split each ray's steps over nseg
+        // wavefronts (about 32 wavefronts per CU in all) and add the partial
+        // sums in a second pass.
+        const int64_t target = (int64_t)ctx->cu_count * 32;
+        int64_t nseg = (target + nwaves - 1) / nwaves;
+        nseg = nseg < 1 ? 1 : (nseg > 8 ? 8 : nseg);
+        const int64_t nblocks = (nwaves * nseg + 3) / 4;
         // 32-bit record offsets; gridDim.z of the pack launch; wavefront index
         const bool fits = nc >= 1 && planes >= 1 && bytes < ((int64_t)1 << 32) && planes <= 65535 &&
-                          nwaves < ((int64_t)1 << 31);
+                          nwaves * nseg < ((int64_t)1 << 31);
         if (fits) {
             int rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, 2 * (bytes / 4));
             if (rc) return rc;
+            const int64_t nout = (int64_t)a.nrows * a.nviews * a.nu_out;
+            if (nseg > 1) {
+                rc = mct::ensure_buffer(ctx, &ctx->fp_partial, &ctx->fp_partial_cap, nseg * nout);
+                if (rc) return rc;
+            }
             float4 *xm = reinterpret_cast<float4 *>(ctx->fp_pack);
             float4 *ym = reinterpret_cast<float4 *>(ctx->fp_pack + bytes / 4);
             const dim3 pgrid((unsigned)((nc + 15) / 16), (unsigned)((nc + 15) / 16), (unsigned)planes);
@@ -581,13 +617,18 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
                 hipLaunchKernelGGL(fp_pack_kernel<true>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves);
+                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
+                                   (int)nseg, ctx->fp_partial);
             } else {
                 hipLaunchKernelGGL(fp_pack_kernel<false>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves);
+                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
+                                   (int)nseg, ctx->fp_partial);
             }
+            if (nseg > 1)
+                hipLaunchKernelGGL(fp_segments_kernel, dim3(mct::blocks_for(nout, 256)), dim3(256), 0,
+                                   ctx->stream, ctx->fp_partial, nout, (int)nseg, out);
             MCT_LAUNCHED();
             ctx->fp_last = MCT_FP_CELLS;
             return MCT_OK;

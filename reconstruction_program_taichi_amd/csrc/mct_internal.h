@@ -21,6 +21,8 @@ struct mct_ctx {
     // packed volume of the forward projector (two orders of bilinear cells)
     float *fp_pack = nullptr;
     int64_t fp_pack_cap = 0;  // floats
+    float *fp_partial = nullptr;  // per-segment partial sums of split rays
+    int64_t fp_partial_cap = 0;
     // partial sums of the deterministic FP64 reductions
     double *partials = nullptr;
     int64_t partials_cap = 0;
