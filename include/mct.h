@@ -233,7 +233,10 @@ int mct_bp_last_kernel(const mct_ctx *ctx);
  * [n_v][n_views][n_u/oversample] (row r = detector row r of v[]).
  * Fan beam (cone == 0) reads slice r for detector row r.
  * ray_mask: NULL or DEVICE [n_v][n_views][n_u/oversample]; rays whose mask is
- * not 1 are left 0 (oversample must be 1).  oversample: OversampleSize. */
+ * not 1 are left 0 (oversample must be 1).  oversample: OversampleSize.
+ * apply_pmatrix: the P-matrix branch of run_mgfpj_v2.py:334-355 (sample l of
+ * ray (u index i, row r, view j) at (l/|d - s|) A_j [i, r, 1] + x_s_j, rotated
+ * by angles[0]); matrix_a and x_s are then required. */
 int mct_forward_project(mct_plan *plan, const float *vol, float *out, const float *ray_mask,
                         int32_t oversample);
 
@@ -259,7 +262,8 @@ int mct_fp_last_kernel(const mct_ctx *ctx);
 /* Mgfbp_ir.GenEffectiveMapForwardProjectionAgent
  * run_mgfbp_iterative_recon_IRN_CGLS.py:679-833: 1 for rays whose samples
  * inside the in-plane FOV disc all stay within the volume's z range, else 0
- * (cone beam; fan beam gives all ones).  mask DEVICE [n_v][n_views][n_u]. */
+ * (cone beam; fan beam gives all ones); P-matrix rays as in
+ * mct_forward_project.  mask DEVICE [n_v][n_views][n_u]. */
 int mct_ray_mask(mct_plan *plan, float *mask);
 
 /* Mgfpj.AddPossionNoise run_mgfpj.py:643-654: N = I0 e^-p; N += g sqrt(N)
@@ -274,6 +278,7 @@ int mct_add_noise(mct_ctx *ctx, float *sgm, int64_t n, float photon_number, uint
  * run_mgfbp_iterative_recon_IRN_CGLS.py:372-474 over ALL views in one launch,
  * then SetTruncatedRegionToZero :477-481: unfiltered back-projection with the
  * ray-length weight; a voxel whose u falls off the detector in any view is 0.
+ * Every ReconViewMode and the P-matrix branch (:423-458).
  * sgm DEVICE FP32 [n_v][n_views][n_u]; ray_mask NULL or the same shape
  * (multiplies sgm); vol DEVICE [img_dim_z][img_dim][img_dim], overwritten. */
 int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, float *vol);

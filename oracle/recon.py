@@ -193,10 +193,22 @@ def filter_vertical(x, kern, det_h, dtype=np.float64):
 
 
 # ---------------------------------------------------------- back-projection
+def _pm_project(pm, j, x, y, z, T):
+    """P-matrix branch of run_mgfbp.py:1133-1141 and :1165-1168 (also
+    run_mgfbp_iterative_recon_IRN_CGLS.py:430-458): the detector indexes of
+    voxel (x, y, z) in view j, summed in the reference's order."""
+    P = [T(p) for p in pm[12 * j:12 * j + 12]]
+    mag = T(1) / (P[8] * x + P[9] * y + P[10] * z + P[11] * T(1))
+    uidx = (P[0] * x + P[1] * y + P[2] * z + P[3] * T(1)) * mag
+    vidx = (P[4] * x + P[5] * y + P[6] * z + P[7] * T(1)) * mag
+    return mag, uidx, vidx
+
+
 def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
     """Mgfbp.BackProjectionPixelDriven (run_mgfbp.py:1062-1187), axial view,
-    no P-matrix.  filt: (rows, views, n_u).  g: dict of geometry scalars and
-    arrays (angles, u, v).  Returns (nz, ny, nx)."""
+    with the P-matrix branch when g["pmatrix"] (12 per view) is given.
+    filt: (rows, views, n_u).  g: dict of geometry scalars and arrays
+    (angles, u, v).  Returns (nz, ny, nx)."""
     T = _t(dtype)
     f = np.asarray(filt, dtype=dtype)
     nx, nz = g["img_dim"], g["img_dim_z"]
@@ -231,15 +243,21 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
         adv = np.abs(dv)
         dh = T(g["det_h"])
     slice_idx = np.broadcast_to(np.arange(nz)[:, None, None], x.shape)
+    pm = g.get("pmatrix")
+    if pm is not None:
+        pm = np.asarray(pm, dtype=dtype)
     for j in range(ang.shape[0]):
         L = sid - x * c[j] - y * s[j]
-        mag = sdd / L
-        yr = (-x) * s[j] + y * c[j]
-        if curved:
-            uu = (sdd * np.arctan2(yr.astype(np.float64), L.astype(np.float64))).astype(dtype)
+        if pm is not None:
+            mag, uidx, vidx_pm = _pm_project(pm, j, x, y, z, T)
         else:
-            uu = mag * yr
-        uidx = (uu - u0) / du
+            mag = sdd / L
+            yr = (-x) * s[j] + y * c[j]
+            if curved:
+                uu = (sdd * np.arctan2(yr.astype(np.float64), L.astype(np.float64))).astype(dtype)
+            else:
+                uu = mag * yr
+            uidx = (uu - u0) / du
         alive &= ~((uidx < 0) | (uidx + T(1) > T(nu - 1)))
         uf = np.clip(np.floor(uidx), 0, nu - 2).astype(np.int64)
         ru = uidx - uf.astype(dtype)
@@ -249,7 +267,7 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
         else:
             w = T(1) / (L * L)
         if cone:
-            vidx = (mag * z - v0) / dh * adv / dv
+            vidx = vidx_pm if pm is not None else (mag * z - v0) / dh * adv / dv
             vff = np.floor(vidx)
             alive &= ~((vff < 0) | (vff + T(1) > T(nrows - 1)))
             vf = np.clip(vff, 0, nrows - 2).astype(np.int64)
@@ -267,7 +285,8 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
 def backproject_ir(sgm, g, ray_mask=None, dtype=np.float32):
     """Mgfbp_ir.BackProjectionPixelDrivenPerView over all views plus
     SetTruncatedRegionToZero (run_mgfbp_iterative_recon_IRN_CGLS.py:372-481),
-    axial view.  sgm: (rows, views, n_u)."""
+    axial view, P-matrix branch when g["pmatrix"] is given.
+    sgm: (rows, views, n_u)."""
     T = _t(dtype)
     f = np.asarray(sgm, dtype=dtype)
     if ray_mask is not None:
@@ -302,6 +321,9 @@ def backproject_ir(sgm, g, ray_mask=None, dtype=np.float32):
         dh = T(g["det_h"])
     czs = g.get("center_z")
     slice_idx = np.broadcast_to(np.arange(nz)[:, None, None], x.shape)
+    pm = g.get("pmatrix")
+    if pm is not None:
+        pm = np.asarray(pm, dtype=dtype)
     for j in range(ang.shape[0]):
         z = zb + (T(czs[j]) if czs is not None else cz)
         L = sid - x * c[j] - y * s[j]
@@ -309,15 +331,18 @@ def backproject_ir(sgm, g, ray_mask=None, dtype=np.float32):
         dys = y - sid * s[j]
         den = np.where(np.abs(dxs) > np.abs(dys), np.abs(dxs), np.abs(dys))
         df = np.sqrt(dxs * dxs + dys * dys + z * z) / den
-        mag = sdd / L
-        uu = mag * ((-x) * s[j] + y * c[j])
-        uidx = (uu - u0) / du
+        if pm is not None:
+            mag, uidx, vidx_pm = _pm_project(pm, j, x, y, z, T)
+        else:
+            mag = sdd / L
+            uu = mag * ((-x) * s[j] + y * c[j])
+            uidx = (uu - u0) / du
         bad = (uidx < 0) | (uidx + T(1) > T(nu - 1))
         flag &= ~bad
         uf = np.clip(np.floor(uidx), 0, nu - 2).astype(np.int64)
         ru = uidx - uf.astype(dtype)
         if cone:
-            vidx = (mag * z - v0) / dh * adv / dv
+            vidx = vidx_pm if pm is not None else (mag * z - v0) / dh * adv / dv
             vff = np.floor(vidx)
             okv = ~((vff < 0) | (vff + T(1) > T(nrows - 1)))
             vf = np.clip(vff, 0, nrows - 2).astype(np.int64)
@@ -352,10 +377,40 @@ def fp_scalars(g, count_mode=0, dtype=np.float32):
     return dict(lmin=lmin, diag=diag, count=count, x0=x0, y0=y0, zbase=zbase)
 
 
+def _pm_rays(g, ang, Ui, R, J, dtype):
+    """P-matrix branch of the forward projector (run_mgfpj_v2.py:334-355,
+    run_mgfpj_v3.py:286-307, run_mgfbp_iterative_recon_IRN_CGLS.py:598-622):
+    sample l of ray (u index, row, view j) sits at
+    s * (A_j [u, v, 1]) + x_s_j with s = l / |detector element - source|,
+    z shifted by the helical offset, then rotated by angles[0].  Returns
+    None without g["matrix_a"] (9 per view) and g["x_s"] (3 per view), else
+    a function of (l, dist, zs) giving (x_rot, y_rot, z_rot)."""
+    if g.get("matrix_a") is None:
+        return None
+    T = _t(dtype)
+    nv = ang.shape[0]
+    A = np.asarray(g["matrix_a"], dtype=dtype).reshape(nv, 9)[J]
+    xs = np.asarray(g["x_s"], dtype=dtype).reshape(nv, 3)[J]
+    fu, fv = Ui.astype(dtype), R.astype(dtype)
+    w = [A[..., 3 * r] * fu + A[..., 3 * r + 1] * fv + A[..., 3 * r + 2] * T(1) for r in range(3)]
+    c0 = T(np.cos(np.float64(ang[0])))
+    s0 = T(np.sin(np.float64(ang[0])))
+
+    def at(l, dist, zs):
+        om = l / dist
+        xp = om * w[0] + xs[..., 0]
+        yp = om * w[1] + xs[..., 1]
+        zp = om * w[2] + xs[..., 2] + zs
+        return xp * c0 - yp * s0, yp * c0 + xp * s0, zp
+
+    return at
+
+
 def forward_project(vol, g, oversample=1, ray_mask=None, count_mode=0, dtype=np.float32,
                     samples_in=False):
     """Mgfpj.ForwardProjectionBilinear + BinSinogram (run_mgfpj.py:463-616),
-    all rows and views; u already oversampled (len = n_u * oversample).
+    all rows and views; u already oversampled (len = n_u * oversample); the
+    P-matrix rays of _pm_rays when g carries matrix_a and x_s.
     vol: (nz, ny, nx).  Returns (rows, views, n_u)."""
     T = _t(dtype)
     vol = np.asarray(vol, dtype=dtype)
@@ -393,13 +448,17 @@ def forward_project(vol, g, oversample=1, ray_mask=None, count_mode=0, dtype=np.
     step, diag, lmin = T(g.get("fpj_step", 0.2)), sc["diag"], sc["lmin"]
     x0, y0 = sc["x0"], sc["y0"]
     flat = vol.reshape(-1)
+    pm_ray = _pm_rays(g, ang, Ui, R, J, dtype)
     for k in range(sc["count"]):
         l = T(k) * step * diag + lmin
-        x = sid + ux * l
-        y = T(0) + uy * l
-        z = zs + uz * l
-        xr = x * ca - y * sa
-        yr = y * ca + x * sa
+        if pm_ray is not None:
+            xr, yr, z = pm_ray(l, dist, zs)
+        else:
+            x = sid + ux * l
+            y = T(0) + uy * l
+            z = zs + uz * l
+            xr = x * ca - y * sa
+            yr = y * ca + x * sa
         xi = np.floor((xr - x0) / pix).astype(np.int64)
         yi = np.floor((yr - y0) / (-pix)).astype(np.int64)
         ok = (xi >= 0) & (xi + 1 < nx) & (yi >= 0) & (yi + 1 < nx)
@@ -442,7 +501,8 @@ def forward_project(vol, g, oversample=1, ray_mask=None, count_mode=0, dtype=np.
 def ray_mask(g, count_mode=0, dtype=np.float32):
     """GenEffectiveMapForwardProjectionAgent
     (run_mgfbp_iterative_recon_IRN_CGLS.py:679-833): 0 when a sample inside
-    the image and its inscribed disc leaves the z range (cone beam)."""
+    the image and its inscribed disc leaves the z range (cone beam); P-matrix
+    rays as in forward_project."""
     T = _t(dtype)
     nx, nz = g["img_dim"], g["img_dim_z"]
     ang = np.asarray(g["angles"], dtype=dtype)
@@ -469,13 +529,17 @@ def ray_mask(g, count_mode=0, dtype=np.float32):
     m = np.ones(Ui.shape, dtype=bool)
     half = T(nx) / T(2)
     step, diag, lmin = T(g.get("fpj_step", 0.2)), sc["diag"], sc["lmin"]
+    pm_ray = _pm_rays(g, ang, Ui, R, J, dtype)
     for k in range(sc["count"]):
         l = T(k) * step * diag + lmin
-        x = sid + ux * l
-        y = uy * l
-        z = zs + uz * l
-        xr = x * ca - y * sa
-        yr = y * ca + x * sa
+        if pm_ray is not None:
+            xr, yr, z = pm_ray(l, dist, zs)
+        else:
+            x = sid + ux * l
+            y = uy * l
+            z = zs + uz * l
+            xr = x * ca - y * sa
+            yr = y * ca + x * sa
         xi = np.floor((xr - sc["x0"]) / pix)
         yi = np.floor((yr - sc["y0"]) / (-pix))
         ok = (xi >= 0) & (xi + 1 < nx) & (yi >= 0) & (yi + 1 < nx)

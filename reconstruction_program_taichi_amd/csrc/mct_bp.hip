@@ -897,6 +897,83 @@ __global__ __launch_bounds__(256) void bp_ir_axial_kernel(BpArgs a) {
         if (k < nk) out[(size_t)(z0 + k) * plane] = alive ? acc[k] : 0.f;
 }
 
+// IR back-projection for every view mode and the P-matrix branch
+// (run_mgfbp_iterative_recon_IRN_CGLS.py:384-474): one thread per voxel,
+// same weight, tests and truncation flag as bp_ir_axial_kernel.
+__global__ __launch_bounds__(256) void bp_ir_generic_kernel(BpArgs a) {
+    const int64_t n = (int64_t)a.nz * a.nx * a.nx;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int ix = (int)(id % a.nx);
+    const int iy = (int)((id / a.nx) % a.nx);
+    const int iz = (int)(id / ((int64_t)a.nx * a.nx));
+    float x, y, zb;
+    voxel_xyz(a, ix, iy, iz, 0.f, x, y, zb);
+    const float *sg = reinterpret_cast<const float *>(a.sino);
+    const size_t row_stride = (size_t)a.nviews * a.nu;
+    const float vmax = (float)(a.nv - 1);
+    float acc = 0.f;
+    bool alive = true;
+    for (int j = 0; j < a.nviews; ++j) {
+        const float4 t = a.vt[j];
+        const float z = zb + (a.center_z ? a.center_z[j] : a.cz);
+        const float L = a.sid - x * t.x - y * t.y;
+        const float dxs = x - a.sid * t.x;
+        const float dys = y - a.sid * t.y;
+        const float den = fabsf(dxs) > fabsf(dys) ? fabsf(dxs) : fabsf(dys);
+        float mag, uidx;
+        const float *P = a.apply_pm ? a.pm + 12 * j : nullptr;
+        if (!a.apply_pm) {
+            mag = a.sdd / L;
+            const float yr = (-x) * t.y + y * t.x;
+            const float u = a.curved ? a.sdd * atan2f(yr, L) : mag * yr;
+            uidx = (u - a.u0) / a.du;
+        } else {
+            mag = 1.0f / (P[8] * x + P[9] * y + P[10] * z + P[11] * 1.0f);
+            uidx = (P[0] * x + P[1] * y + P[2] * z + P[3] * 1.0f) * mag;
+        }
+        if (uidx < 0.f || uidx + 1.f > (float)(a.nu - 1)) {
+            alive = false;
+            break;
+        }
+        int uf = (int)floorf(uidx);
+        uf = uf > a.nu - 2 ? a.nu - 2 : uf;
+        const float ru = uidx - (float)uf;
+        const float wgt = sqrtf(dxs * dxs + dys * dys + z * z) / den;
+        const size_t vb = (size_t)j * a.nu + uf;
+        float val;
+        if (a.cone) {
+            const float vidx = a.apply_pm ? (P[4] * x + P[5] * y + P[6] * z + P[7] * 1.0f) * mag
+                                          : (mag * z - a.v0) / a.det_h * a.adv / a.dv;
+            const float vff = floorf(vidx);
+            if (vff < 0.f || vff + 1.f > vmax) continue;
+            const int vf = (int)vff;
+            const float rv = vidx - vff;
+            const size_t r0 = (size_t)vf * row_stride + vb, r1 = r0 + row_stride;
+            float s00 = sg[r0], s01 = sg[r0 + 1], s10 = sg[r1], s11 = sg[r1 + 1];
+            if (a.mask) {
+                s00 *= a.mask[r0];
+                s01 *= a.mask[r0 + 1];
+                s10 *= a.mask[r1];
+                s11 *= a.mask[r1 + 1];
+            }
+            const float p0 = s00 * (1.f - ru) + s01 * ru;
+            const float p1 = s10 * (1.f - ru) + s11 * ru;
+            val = (1.f - rv) * p0 + rv * p1;
+        } else {
+            const size_t r0 = (size_t)iz * row_stride + vb;
+            float s0 = sg[r0], s1 = sg[r0 + 1];
+            if (a.mask) {
+                s0 *= a.mask[r0];
+                s1 *= a.mask[r0 + 1];
+            }
+            val = (1.f - ru) * s0 + ru * s1;
+        }
+        acc += val * wgt * a.pix;
+    }
+    a.vol[id] = alive ? acc : 0.f;
+}
+
 BpArgs make_args(const mct_plan *p) {
     const mct_geometry &g = p->g;
     BpArgs a = {};
@@ -1241,17 +1318,22 @@ int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, 
     MCT_CHECK(plan && sgm && vol, "NULL argument");
     const mct_geometry &g = plan->g;
     mct_ctx *ctx = plan->ctx;
-    MCT_CHECK(g.view_mode == 1 && !g.apply_pmatrix,
-              "IR back-projection supports the axial view without P-matrix");
     if (!g.cone)
         MCT_CHECK(g.n_v == g.img_dim_z, "fan beam: one detector row per slice required");
-    else
+    else if (!g.apply_pmatrix)
         MCT_CHECK(g.det_h > 0.f, "cone beam needs det_h > 0");
     BpArgs a = make_args(plan);
     a.sino = sgm;
     a.mask = ray_mask;
     a.layout = MCT_PLAIN;
     a.vol = vol;
+    if (g.view_mode != 1 || g.apply_pmatrix) {
+        const int64_t n = (int64_t)g.img_dim_z * g.img_dim * g.img_dim;
+        hipLaunchKernelGGL(bp_ir_generic_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0,
+                           ctx->stream, a);
+        MCT_LAUNCHED();
+        return MCT_OK;
+    }
     const int zc = 8;
     a.tiles_x = (g.img_dim + 15) / 16;
     a.tiles_y = a.tiles_x;

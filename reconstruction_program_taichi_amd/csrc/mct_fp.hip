@@ -29,18 +29,29 @@ struct FpArgs {
     int nx, nz, nu_os, nu_out, nrows, nviews, os, count;
     float pix, inv_pix, vh, inv_vh, x0, y0, zbase, cz;
     float sid, sdd, lmin, step, diag, zpv;
-    int cone, curved;
+    int cone, curved, apply_pm;
+    float c0, s0;  // cos / sin of angles[0] (P-matrix branch)
     const float *u, *v, *center_z;
+    const float *pm_a, *pm_xs;  // P-matrix branch: A (9 per view), x_s (3 per view)
     const float2 *ft;  // {cos, sin} of the view angle
     const float *vol;
     const float *mask;  // NULL or [nrows][nviews][nu_out]
     float *out;         // [nrows][nviews][nu_out]
 };
 
-// Source and unit direction of ray (u index ui, row vi, view j):
-// run_mgfpj.py:531-551.
-__device__ __forceinline__ void ray_setup(const FpArgs &a, int ui, int vi, int j, float &sz,
-                                          float &ux, float &uy, float &uz) {
+// A ray in the rotated (volume) frame: sample l sits at
+// (ax + l bx, ay + l by, az + l bz).
+struct Ray {
+    float ax, ay, az, bx, by, bz;
+};
+
+// Ray of u index ui, row vi, view j.  Standard geometry (run_mgfpj.py:
+// 531-551, 563-567): source (sid, 0, zshift), unit direction towards the
+// detector element, rotated by the view angle.  P-matrix branch
+// (run_mgfpj_v2.py:334-355): sample l sits at (l / dist) * A_j [ui, vi, 1]
+// + x_s_j (z plus the helical shift), rotated by angles[0]; dist is the
+// standard source-to-element distance, as in the reference.
+__device__ __forceinline__ Ray make_ray(const FpArgs &a, int ui, int vi, int j) {
     float dx, dy;
     if (a.curved) {
         float gp = (-a.u[ui]) / a.sdd;
@@ -50,23 +61,42 @@ __device__ __forceinline__ void ray_setup(const FpArgs &a, int ui, int vi, int j
         dx = -(a.sdd - a.sid);
         dy = a.u[ui];
     }
-    float zshift = a.zpv * (float)j;
-    float dz = a.v[vi] + zshift;
-    sz = zshift;
-    float ex = dx - a.sid, ey = dy - 0.0f, ez = dz - sz;
-    float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-    ux = ex / dist;
-    uy = ey / dist;
-    uz = ez / dist;
+    const float zshift = a.zpv * (float)j;
+    const float dz = a.v[vi] + zshift;
+    const float ex = dx - a.sid, ey = dy - 0.0f, ez = dz - zshift;
+    const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+    Ray r;
+    if (!a.apply_pm) {
+        const float ux = ex / dist, uy = ey / dist, uz = ez / dist;
+        const float2 cs = a.ft[j];
+        r.ax = a.sid * cs.x;
+        r.ay = a.sid * cs.y;
+        r.az = zshift;
+        r.bx = ux * cs.x - uy * cs.y;
+        r.by = uy * cs.x + ux * cs.y;
+        r.bz = uz;
+    } else {
+        const float *A = a.pm_a + 9 * j;
+        const float *xs = a.pm_xs + 3 * j;
+        const float fu = (float)ui, fv = (float)vi;
+        const float w0 = A[0] * fu + A[1] * fv + A[2];
+        const float w1 = A[3] * fu + A[4] * fv + A[5];
+        const float w2 = A[6] * fu + A[7] * fv + A[8];
+        r.ax = xs[0] * a.c0 - xs[1] * a.s0;
+        r.ay = xs[1] * a.c0 + xs[0] * a.s0;
+        r.az = xs[2] + zshift;
+        r.bx = (w0 * a.c0 - w1 * a.s0) / dist;
+        r.by = (w1 * a.c0 + w0 * a.s0) / dist;
+        r.bz = w2 / dist;
+    }
+    return r;
 }
 
 // Range [k_lo, k_hi) of sample indexes that can fall inside the volume's
-// bounding box (plus one step of margin on each side).
-__device__ __forceinline__ void clip_range(const FpArgs &a, float sz, float ux, float uy, float uz,
-                                           float c, float s, float z0, int &k_lo, int &k_hi) {
-    // rotated source and direction (FP64)
-    double px = (double)a.sid * c - 0.0 * s, py = 0.0 * c + (double)a.sid * s, pz = sz;
-    double dx = (double)ux * c - (double)uy * s, dy = (double)uy * c + (double)ux * s, dz = uz;
+// bounding box (plus one step of margin on each side); with cone = false
+// only the in-plane box.
+__device__ __forceinline__ void clip_range(const FpArgs &a, const Ray &r, float z0, bool cone,
+                                           int &k_lo, int &k_hi) {
     double lo = -1e300, hi = 1e300;
     // box in rotated coordinates: x in [x0, x0+(nx-1)p], y in [y0-(nx-1)p, y0]
     double bx0 = a.x0, bx1 = (double)a.x0 + (double)(a.nx - 1) * a.pix;
@@ -81,9 +111,9 @@ __device__ __forceinline__ void clip_range(const FpArgs &a, float sz, float ux, 
         lo = t0 > lo ? t0 : lo;
         hi = t1 < hi ? t1 : hi;
     };
-    slab(px, dx, bx0, bx1);
-    slab(py, dy, by0, by1);
-    if (a.cone) slab(pz, dz, (double)z0, (double)z0 + (double)(a.nz - 1) * a.vh);
+    slab(r.ax, r.bx, bx0, bx1);
+    slab(r.ay, r.by, by0, by1);
+    if (cone) slab(r.az, r.bz, (double)z0, (double)z0 + (double)(a.nz - 1) * a.vh);
     if (!(lo <= hi)) {
         k_lo = k_hi = 0;
         return;
@@ -99,24 +129,18 @@ __device__ __forceinline__ void clip_range(const FpArgs &a, float sz, float ux, 
 
 // Line integral of one ray: run_mgfpj.py:554-605.
 __device__ float trace(const FpArgs &a, int ui, int vi, int j) {
-    float sz, ux, uy, uz;
-    ray_setup(a, ui, vi, j, sz, ux, uy, uz);
-    const float2 cs = a.ft[j];
-    const float c = cs.x, s = cs.y;
+    const Ray r = make_ray(a, ui, vi, j);
     const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
     int k_lo, k_hi;
-    clip_range(a, sz, ux, uy, uz, c, s, z0, k_lo, k_hi);
-    // rotated ray: P(l) = R (S + l U)
-    const float ax = a.sid * c, ay = a.sid * s;
-    const float bx = ux * c - uy * s, by = uy * c + ux * s;
+    clip_range(a, r, z0, a.cone, k_lo, k_hi);
     const float dl = a.step * a.diag;
     const int nx = a.nx;
     const size_t plane = (size_t)nx * nx;
     float sum = 0.f;
     for (int k = k_lo; k < k_hi; ++k) {
         float l = (float)k * a.step * a.diag + a.lmin;
-        float xr = fmaf(l, bx, ax);
-        float yr = fmaf(l, by, ay);
+        float xr = fmaf(l, r.bx, r.ax);
+        float yr = fmaf(l, r.by, r.ay);
         float fx = (xr - a.x0) * a.inv_pix;
         float fy = (yr - a.y0) * (-a.inv_pix);
         float flx = floorf(fx), fly = floorf(fy);
@@ -124,7 +148,7 @@ __device__ float trace(const FpArgs &a, int ui, int vi, int j) {
         if (xi < 0 || xi + 1 >= nx || yi < 0 || yi + 1 >= nx) continue;
         float xw = fx - flx, yw = fy - fly;
         if (a.cone) {
-            float zr = fmaf(l, uz, sz);
+            float zr = fmaf(l, r.bz, r.az);
             float fz = (zr - z0) * a.inv_vh;
             float flz = floorf(fz);
             int zi = (int)flz;
@@ -272,13 +296,10 @@ template <bool CONE>
 __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j,
                              __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy, uint32_t pz,
                              int seg, int nseg) {
-    float sz, ux, uy, uz;
-    ray_setup(a, active ? ui : 0, vi, j, sz, ux, uy, uz);
-    const float2 cs = a.ft[j];
-    const float c = cs.x, s = cs.y;
+    const Ray r = make_ray(a, active ? ui : 0, vi, j);
     const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
     int k_lo, k_hi;
-    clip_range(a, sz, ux, uy, uz, c, s, z0, k_lo, k_hi);
+    clip_range(a, r, z0, CONE, k_lo, k_hi);
     if (!active || k_hi <= k_lo) {
         k_lo = 0x7fffffff;
         k_hi = -1;
@@ -291,8 +312,6 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
         k_lo = s0;
         k_hi = s1;
     }
-    const float ax = a.sid * c, ay = a.sid * s;
-    const float bx = ux * c - uy * s, by = uy * c + ux * s;
     const float dl = a.step * a.diag;
     const int nc = a.nx - 1;
     constexpr uint32_t kRec = CONE ? 32u : 16u;
@@ -306,8 +325,8 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
         for (int i = 0; i < U; ++i) {
             const int k = k0 + i;
             const float l = (float)k * a.step * a.diag + a.lmin;
-            const float xr = fmaf(l, bx, ax);
-            const float yr = fmaf(l, by, ay);
+            const float xr = fmaf(l, r.bx, r.ax);
+            const float yr = fmaf(l, r.by, r.ay);
             const float fx = (xr - a.x0) * a.inv_pix;
             const float fy = (yr - a.y0) * (-a.inv_pix);
             const float flx = floorf(fx), fly = floorf(fy);
@@ -319,7 +338,7 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
             yi = yi < 0 ? 0 : (yi > nc - 1 ? nc - 1 : yi);
             uint32_t cell = __umul24((uint32_t)xi, sx) + __umul24((uint32_t)yi, sy);
             if (CONE) {
-                const float zr = fmaf(l, uz, sz);
+                const float zr = fmaf(l, r.bz, r.az);
                 const float fz = (zr - z0) * a.inv_vh;
                 const float flz = floorf(fz);
                 int zi = (int)flz;
@@ -367,9 +386,16 @@ __global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *x
     const bool masked = inside && a.mask && a.mask[oidx] != 1.0f;
     const bool active = inside && !masked;
     // neighbouring detector columns step along (-sin, cos) of the view in
-    // the volume: the y-major order when that is closer to the y axis
-    const float2 cs = a.ft[j];
-    const bool ymaj = fabsf(cs.x) >= fabsf(cs.y);
+    // the volume (P-matrix: along the rotated first column of A): the
+    // y-major order when that is closer to the y axis
+    bool ymaj;
+    if (!a.apply_pm) {
+        const float2 cs = a.ft[j];
+        ymaj = fabsf(cs.x) >= fabsf(cs.y);
+    } else {
+        const float *A = a.pm_a + 9 * j;
+        ymaj = fabsf(A[3] * a.c0 + A[0] * a.s0) >= fabsf(A[0] * a.c0 - A[3] * a.s0);
+    }
     const uint32_t nc = (uint32_t)(a.nx - 1);
     const uint32_t sx = ymaj ? nc : 1u, sy = ymaj ? 1u : nc;
     const uint32_t pz = CONE ? nc * nc : (uint32_t)vi * nc * nc;
@@ -404,29 +430,23 @@ __global__ __launch_bounds__(256) void ray_mask_kernel(FpArgs a) {
     int j = blockIdx.z;
     if (ui >= a.nu_out || vi >= a.nrows) return;
     size_t oidx = ((size_t)vi * a.nviews + j) * a.nu_out + ui;
-    float sz, ux, uy, uz;
-    ray_setup(a, ui, vi, j, sz, ux, uy, uz);
-    const float2 cs = a.ft[j];
-    const float c = cs.x, s = cs.y;
+    const Ray r = make_ray(a, ui, vi, j);
     const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
     int k_lo, k_hi;
-    FpArgs b = a;
-    b.cone = 0;  // clip in-plane only: the test fires when z leaves the range
-    clip_range(b, sz, ux, uy, uz, c, s, z0, k_lo, k_hi);
-    const float ax = a.sid * c, ay = a.sid * s;
-    const float bx = ux * c - uy * s, by = uy * c + ux * s;
+    // clip in-plane only: the test fires when z leaves the range
+    clip_range(a, r, z0, false, k_lo, k_hi);
     const float half = (float)a.nx / 2.0f;
     float m = 1.0f;
     for (int k = k_lo; k < k_hi; ++k) {
         float l = (float)k * a.step * a.diag + a.lmin;
-        float xr = fmaf(l, bx, ax);
-        float yr = fmaf(l, by, ay);
+        float xr = fmaf(l, r.bx, r.ax);
+        float yr = fmaf(l, r.by, r.ay);
         int xi = (int)floorf((xr - a.x0) * a.inv_pix);
         int yi = (int)floorf((yr - a.y0) * (-a.inv_pix));
         if (xi < 0 || xi + 1 >= a.nx || yi < 0 || yi + 1 >= a.nx) continue;
         float dxi = (float)xi - half, dyi = (float)yi - half;
         if (dxi * dxi + dyi * dyi < half * half) {
-            float zr = fmaf(l, uz, sz);
+            float zr = fmaf(l, r.bz, r.az);
             int zi = (int)floorf((zr - z0) * a.inv_vh);
             if (zi < 0 || zi + 1 >= a.nz) {
                 m = 0.f;
@@ -553,6 +573,13 @@ int fill_args(mct_plan *p, FpArgs &a, int count_mode) {
     a.center_z = p->center_z;
     a.ft = p->fp_view;
     a.zpv = p->zpv;
+    a.apply_pm = g.apply_pmatrix;
+    a.c0 = p->cos_a0;
+    a.s0 = p->sin_a0;
+    a.pm_a = p->matrix_a;
+    a.pm_xs = p->x_s;
+    MCT_CHECK(!g.apply_pmatrix || (p->matrix_a && p->x_s),
+              "P-matrix forward projection needs matrix_a and x_s (9 and 3 floats per view)");
     return MCT_OK;
 }
 
@@ -564,7 +591,6 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
                         int32_t oversample) {
     MCT_CHECK(plan && vol && out, "NULL argument");
     const mct_geometry &g = plan->g;
-    MCT_CHECK(!g.apply_pmatrix, "P-matrix forward projection is not implemented");
     MCT_CHECK(oversample >= 1, "oversample must be >= 1 (got %d)", oversample);
     MCT_CHECK(g.n_u % oversample == 0, "n_u (%d) is not a multiple of oversample (%d)", g.n_u,
               oversample);
@@ -574,7 +600,8 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
     MCT_CHECK(g.cone || g.n_v <= g.img_dim_z, "fan beam: row %d has no slice (%d slices)", g.n_v - 1,
               g.img_dim_z);
     FpArgs a;
-    fill_args(plan, a, plan->fp_count_mode);
+    int rc0 = fill_args(plan, a, plan->fp_count_mode);
+    if (rc0) return rc0;
     a.os = oversample;
     a.nu_os = g.n_u;
     a.nu_out = g.n_u / oversample;
@@ -656,10 +683,10 @@ int mct_ray_mask(mct_plan *plan, float *mask) {
         MCT_LAUNCHED();
         return MCT_OK;
     }
-    MCT_CHECK(!g.apply_pmatrix, "P-matrix ray mask is not implemented");
     MCT_CHECK(g.vox_h > 0.f, "cone beam needs vox_h > 0");
     FpArgs a;
-    fill_args(plan, a, plan->fp_count_mode);
+    int rc0 = fill_args(plan, a, plan->fp_count_mode);
+    if (rc0) return rc0;
     a.os = 1;
     a.nu_os = g.n_u;
     a.nu_out = g.n_u;

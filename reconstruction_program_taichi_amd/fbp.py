@@ -30,6 +30,7 @@ import numpy as np
 import torch
 
 from . import geometry as geo
+from . import pmatrix as pmx
 from .config import (PI, ReadConfigFile, fail, first_key, imreadRaw, imwriteRaw, imwriteTiff,
                      is_num, save_jsonc)
 from .device import Context, Plan
@@ -92,6 +93,15 @@ class Mgfbp:
             os.makedirs(folder, exist_ok=True)
             save_jsonc(folder + "/config_mgfbp.jsonc", cfg)
             print("Modified config files are saved to %s folder." % folder)
+            if self.bool_apply_pmatrix:
+                # run_mgfbp.py:698-718
+                for name, arr in (("pmatrix_file", self.array_pmatrix),
+                                  ("sid_file", self.source_isocenter_dis_each_view),
+                                  ("sdd_file", self.source_det_dis_each_view),
+                                  ("det_offset_horizontal_file", np.squeeze(self.det_offset_horizontal_each_view)),
+                                  ("det_offset_vertical_file", np.squeeze(self.det_offset_vertical_each_view)),
+                                  ("scan_angle_file", np.squeeze(self.scan_angle_each_view))):
+                    save_jsonc(folder + f"/{name}.jsonc", {"Value": arr.tolist()})
 
     # ------------------------------------------------------------ config
     def _parse_io(self, cfg):
@@ -299,8 +309,78 @@ class Mgfbp:
         else:
             self.convert_to_HU = False
 
+    def _parse_pmatrix(self, cfg):
+        """P-matrix keys and normalisation (run_mgfbp.py:551-627): read
+        PMatrixFile's 'Value' (12 floats per view), derive the scan
+        parameters from the matrices (written back into the config dict, as
+        the reference does), then adapt the matrices to the scan's pixel
+        pitch and vertical reconstruction range."""
+        d = ReadConfigFile(cfg["PMatrixFile"])
+        if "Value" not in d:
+            fail("ERROR: PMatrixFile has no member named 'Value'!")
+        val = d["Value"]
+        if not isinstance(val, list):
+            fail("ERROR: PMatrixFile.Value is not an array")
+        if len(val) != self.view_num * 12:
+            fail(f"ERROR: view number is {self.view_num:d} while pmatrix has {len(val):d} elements!")
+        self.array_pmatrix = np.array(val, dtype=np.float32)
+        self.bool_apply_pmatrix = 1
+        print("--PMatrix applied")
+        self.modify_pmatrix_to_standard_form = cfg.get("ModifyPMatrixToStandardForm", False)
+        if not isinstance(self.modify_pmatrix_to_standard_form, bool):
+            fail("ERROR: ModifyPMatrixToStandardForm must be True or False!")
+        for key, attr, default, what in (
+                ("PMatrixDetectorElementWidth", "pmatrix_elem_width", self.det_elem_width, "width"),
+                ("PMatrixDetectorElementHeight", "pmatrix_elem_height", self.det_elem_height, "height")):
+            if key in cfg:
+                print(f"--PMatrix detector pixel {what} is different from the CT scan")
+                v = cfg[key]
+                if not is_num(v):
+                    fail(f"ERROR: {key} must be a number!")
+                if v <= 0:
+                    fail(f"ERROR: {key} must be positive!")
+                setattr(self, attr, v)
+            else:
+                setattr(self, attr, default)
+        r = pmx.source_trajectory(self.array_pmatrix, self.view_num, self.modify_pmatrix_to_standard_form,
+                                  self.pmatrix_elem_width, self.pmatrix_elem_height,
+                                  self.det_elem_count_horizontal, self.det_elem_width,
+                                  self.det_elem_count_vertical, self.det_elem_height)
+        self.array_pmatrix = r["pmatrix"]
+        self.positive_u_is_positive_y = r["positive_u_is_positive_y"]
+        self.positive_v_is_positive_z = r["positive_v_is_positive_z"]
+        self.det_offset_horizontal = r["det_offset_horizontal"]
+        self.det_offset_vertical = r["det_offset_vertical"]
+        self.source_isocenter_dis = r["source_isocenter_dis"]
+        self.source_det_dis = r["source_det_dis"]
+        self.source_isocenter_dis_each_view = r["sid_each"]
+        self.source_det_dis_each_view = r["sdd_each"]
+        self.det_offset_horizontal_each_view = r["det_offset_horizontal_each"]
+        self.det_offset_vertical_each_view = r["det_offset_vertical_each"]
+        self.scan_angle_each_view = r["scan_angle_each"]
+        print("Parameters are updated from PMatrix:")
+        print("Mean Offset values are %.2f mm and %.2f mm for horizontal and vertical direction respectively;"
+              % (self.det_offset_horizontal, self.det_offset_vertical))
+        print("Mean Source to Isocenter Distance is %.2f mm;" % self.source_isocenter_dis)
+        print("Mean Source to Detector Distance is %.2f mm;" % self.source_det_dis)
+        if self.short_scan:
+            self.total_scan_angle = r["scan_sum"] / (self.view_num - 1) * self.view_num
+            print("Total Scan Angle is %.2f degrees." % (self.total_scan_angle / PI * 180.0))
+        else:
+            print("Total Scan Angle is not updated for full scan.")
+        cfg["SourceIsocenterDistance"] = self.source_isocenter_dis
+        cfg["SourceDetectorDistance"] = self.source_det_dis
+        cfg["DetectorOffsetHorizontal"] = self.det_offset_horizontal
+        cfg["DetectorOffsetVertical"] = self.det_offset_vertical
+        cfg["TotalScanAngle"] = self.total_scan_angle / PI * 180.0
+        self.array_pmatrix = pmx.pixel_size(self.array_pmatrix, self.view_num, self.det_elem_width,
+                                            self.det_elem_height, self.pmatrix_elem_width,
+                                            self.pmatrix_elem_height)
+        self.array_pmatrix = pmx.vertical_recon_range(self.array_pmatrix, self.view_num,
+                                                      self.det_elem_vertical_recon_range_begin)
+
     def _parse_cone(self, cfg):
-        """run_mgfbp.py:495-672 (the P-matrix branch is not supported)."""
+        """run_mgfbp.py:495-672, P-matrix branch included."""
         self.cone_beam = cfg.get("ConeBeam", False)
         if not isinstance(self.cone_beam, bool):
             fail("ERROR: ConeBeam must be true or false!")
@@ -335,14 +415,15 @@ class Mgfbp:
             fail("ERROR: VoxelHeight (ImageSliceThickness) must be positive!")
         self.img_voxel_height = vh
         if "PMatrixFile" in cfg:
-            fail("ERROR: PMatrixFile is not supported by this build!")
-        k, off = first_key(cfg, "SliceOffCenter", "DetectorOffsetVertical")
-        if k is None:
-            off = 0
-            print("Warning: Can not find vertical detector offset for cone beam recon; Using default value 0")
-        if not is_num(off):
-            fail("ERROR: DetectorOffsetVertical (SliceOffCenter) must be a number!")
-        self.det_offset_vertical = off
+            self._parse_pmatrix(cfg)
+        else:
+            k, off = first_key(cfg, "SliceOffCenter", "DetectorOffsetVertical")
+            if k is None:
+                off = 0
+                print("Warning: Can not find vertical detector offset for cone beam recon; Using default value 0")
+            if not is_num(off):
+                fail("ERROR: DetectorOffsetVertical (SliceOffCenter) must be a number!")
+            self.det_offset_vertical = off
         if "ImageCenterZ" in cfg:
             self.img_center_z = cfg["ImageCenterZ"]
             if not is_num(self.img_center_z):
@@ -377,7 +458,8 @@ class Mgfbp:
             sdd=self.source_det_dis, total_scan_angle=self.total_scan_angle, angles=ang, u=u, v=v,
             cx=self.img_center_x, cy=self.img_center_y, cz=self.img_center_z, img_rot=self.img_rot,
             view_mode=self.recon_view_mode, short_scan=bool(self.short_scan), cone=self.cone_beam,
-            curved=self.curved_dect, n_v_full=self.det_elem_count_vertical, det_h=self.det_elem_height)
+            curved=self.curved_dect, n_v_full=self.det_elem_count_vertical, det_h=self.det_elem_height,
+            pmatrix=self.array_pmatrix if self.bool_apply_pmatrix else None)
 
     def recon_kernel(self) -> np.ndarray | None:
         """InitializeReconKernel (run_mgfbp.py:1261-1279)."""

@@ -123,6 +123,10 @@ class GeometrySpec:
     helical_pitch: float = 0.0
     pmatrix: np.ndarray | None = None
     center_z: np.ndarray | None = None
+    # forward-projection P-matrix branch: A = inv(P[:, :3]) and the source
+    # x_s per view (run_mgfpj_v2.py:218-227)
+    matrix_a: np.ndarray | None = None
+    x_s: np.ndarray | None = None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -163,6 +167,8 @@ class GeometrySpec:
         g.angles, g.u, g.v = arr(self.angles), arr(self.u), arr(self.v)
         g.pmatrix = arr(self.pmatrix)
         g.center_z = arr(self.center_z)
+        g.matrix_a = arr(self.matrix_a)
+        g.x_s = arr(self.x_s)
         return g
 
 

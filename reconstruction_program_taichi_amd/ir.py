@@ -33,6 +33,7 @@ import numpy as np
 import torch
 
 from . import geometry as geo
+from . import pmatrix as pmx
 from .config import PI, ReadConfigFile, fail, imreadRaw, imwriteRaw, imwriteTiff, is_num
 from .fbp import Mgfbp
 
@@ -95,6 +96,9 @@ class Mgfbp_ir(Mgfbp):
         if self.oversample_size != 1:
             fail("ERROR: OversampleSize must be 1 for iterative reconstruction!")
         self.fpj_step_size = cfg.get("ForwardProjectionStepSize", 0.2)
+        if self.bool_apply_pmatrix:  # run_mgfpj_v2.py:211-214
+            self.matrix_A_each_view, self.x_s_each_view = pmx.matrix_a_and_source(self.array_pmatrix,
+                                                                                  self.view_num)
         if "HelicalPitch" in cfg:
             self.helical_scan, self.helical_pitch = True, cfg["HelicalPitch"]
         else:
@@ -187,7 +191,11 @@ class Mgfbp_ir(Mgfbp):
             short_scan=bool(self.short_scan), cone=self.cone_beam, curved=self.curved_dect,
             n_v_full=self.det_elem_count_vertical, det_h=self.det_elem_height,
             fpj_step=self.fpj_step_size, fpj_count_mode=0, helical=self.helical_scan,
-            helical_pitch=self.helical_pitch, center_z=self.array_img_center_z)
+            helical_pitch=self.helical_pitch, center_z=self.array_img_center_z,
+            pmatrix=self.array_pmatrix if self.bool_apply_pmatrix else None,
+            matrix_a=self.matrix_A_each_view if self.bool_apply_pmatrix else None,
+            x_s=self.x_s_each_view if self.bool_apply_pmatrix else None,
+            view_mode=self.recon_view_mode)
 
     def ir_plan(self):
         from .device import Plan

@@ -10,7 +10,7 @@ import scipy.sparse as sp
 
 from oracle import cbp, recon
 from reconstruction_program_taichi_amd import geometry as geo
-from tests._cases import cone_case, fan_case, oracle_dict, rel_l2
+from tests._cases import cone_case, fan_case, matrix_a_for, oracle_dict, pmatrix_for, rel_l2
 
 
 # ------------------------------------------------------------------ taps
@@ -494,3 +494,68 @@ def test_tv_weight_and_value():
     x2 = x[:2]
     mag2 = np.sqrt(ux[:2] ** 2 + uy[:2] ** 2 + beta)
     assert np.allclose(recon.tv_weight(x2, beta), 1 / mag2)
+
+
+# ------------------------------------------------------------ P-matrix
+def _pm_spec(rot=0.0, off_v=0.0):
+    # pitches and offsets exact in FP32, so u and v are exactly linear in the index
+    spec = cone_case(nx=24, nz=12, nv=24, nu=60, nrows=30, pix=0.5, vh=0.5, w=0.5, h=0.5,
+                     sid=150.0, sdd=300.0, off_v=off_v)
+    if rot:
+        spec.img_rot = rot
+        spec.angles = geo.angle_array(spec.n_views, spec.total_scan_angle, rot)
+    return spec
+
+
+@pytest.mark.parametrize("rot", [0.0, 0.375])
+def test_pmatrix_bp_equals_standard_bp(rot):
+    """The P-matrix branch of the back-projector with the P-matrices of the
+    standard circular geometry (built from first principles) reproduces the
+    standard branch: same masks, same values (FP64).  A swapped P row, a
+    wrong sign or a transposed matrix moves every voxel's projection."""
+    spec = _pm_spec(rot, off_v=0.75)
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(11)
+    filt = rng.standard_normal((spec.n_v, spec.n_views, spec.n_u))
+    ref, alive = recon.backproject_fbp(filt, g, np.float64, return_mask=True)
+    gp = dict(g, pmatrix=pmatrix_for(spec))
+    got, alive_p = recon.backproject_fbp(filt, gp, np.float64, return_mask=True)
+    assert np.array_equal(alive, alive_p) and alive.mean() > 0.3
+    assert rel_l2(got, ref) <= 1e-9
+    ir = recon.backproject_ir(filt, g, dtype=np.float64)
+    ir_p = recon.backproject_ir(filt, gp, dtype=np.float64)
+    assert rel_l2(ir_p, ir) <= 1e-9 and np.array_equal(ir != 0, ir_p != 0)
+
+
+@pytest.mark.parametrize("rot", [0.0, 0.375])
+def test_pmatrix_fp_and_mask_equal_standard(rot):
+    """Forward projection and ray mask through the P-matrix rays
+    (s A_j [u, v, 1] + x_s_j, rotated by angles[0]) with A, x_s of the
+    standard geometry equal the standard rays (FP64)."""
+    spec = _pm_spec(rot)
+    spec.fpj_step = 0.5
+    g = oracle_dict(spec)
+    vol = np.random.default_rng(12).uniform(0, 1, (spec.img_dim_z, spec.img_dim, spec.img_dim))
+    # the forward projector's frame: rotation by angles[0] happens after A
+    A, xs = matrix_a_for(pmatrix_for(spec, rot=float(spec.angles[0])), spec.n_views)
+    gp = dict(g, matrix_a=A, x_s=xs)
+    ref = recon.forward_project(vol, g, dtype=np.float64)
+    got = recon.forward_project(vol, gp, dtype=np.float64)
+    assert rel_l2(got, ref) <= 1e-9 and (ref > 0).mean() > 0.3
+    m = recon.ray_mask(g, dtype=np.float64)
+    mp = recon.ray_mask(gp, dtype=np.float64)
+    assert np.array_equal(m, mp) and 0.05 < (m == 0).mean() < 0.95
+
+
+def test_pmatrix_fp32_reference_order_close_to_standard():
+    """FP32 reference order: the P-matrix branch differs from the standard
+    one only by rounding of the matrices and the operation order."""
+    spec = _pm_spec()
+    g = oracle_dict(spec)
+    filt = np.random.default_rng(13).standard_normal((spec.n_v, spec.n_views, spec.n_u))
+    ref, alive = recon.backproject_fbp(filt, g, np.float32, return_mask=True)
+    gp = dict(g, pmatrix=pmatrix_for(spec).astype(np.float32))
+    got, alive_p = recon.backproject_fbp(filt, gp, np.float32, return_mask=True)
+    both = alive & alive_p
+    assert (alive != alive_p).mean() < 1e-3
+    assert rel_l2(got[both], ref[both]) <= 1e-5
