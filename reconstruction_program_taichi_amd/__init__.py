@@ -10,7 +10,7 @@ from .config import (PI, ReadConfigFile, XuLeastSquareSolution, imreadRaw, load_
 
 __all__ = ["PI", "ReadConfigFile", "XuLeastSquareSolution", "imreadRaw", "load_jsonc",
            "remove_comments", "save_jsonc", "run_mgfbp", "Mgfbp", "run_mgfpj", "Mgfpj",
-           "run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"]
+           "run_mgfpj_v2", "run_mgfpj_v3", "Mgfpj_v3", "run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"]
 
 
 def __getattr__(name):
@@ -20,6 +20,12 @@ def __getattr__(name):
     if name in ("run_mgfpj", "Mgfpj"):
         from . import fpj
         return getattr(fpj, name)
+    if name == "run_mgfpj_v2":
+        from . import fpj_v2
+        return fpj_v2.run_mgfpj_v2
+    if name in ("run_mgfpj_v3", "Mgfpj_v3"):
+        from . import fpj_v3
+        return getattr(fpj_v3, name)
     if name in ("run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"):
         from . import ir
         return getattr(ir, name)
