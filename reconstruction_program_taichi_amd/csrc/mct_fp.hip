@@ -368,18 +368,35 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
 
 // fp_kernel on cells; xm / ym: the two record orders, `bytes` each (under
 // 4 GiB).  One wavefront traces 64 neighbouring columns of one detector row
-// and view; wavefronts are numbered column block fastest, then view, then
-// row (nwaves in all), four to a workgroup, so the views in flight share
-// detector rows and a single-row (fan) problem still fills every wavefront.
+// and view.  rows4 = 1 (four or more rows): a workgroup's four wavefronts take
+// four neighbouring rows of the same columns and view, workgroups numbered
+// column block fastest, then view, then band of four rows; rows4 = 0: the
+// wavefronts themselves are numbered column block fastest, then view, then
+// row, so a single-row (fan slice) problem still fills every wavefront.
+// nwaves counts the numbered wavefronts (idle ones of a partial band
+// included); each segment of a split ray range repeats the numbering.
 template <bool CONE>
 __global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *xm, const float4 *ym,
                                                        uint32_t bytes, int nub, int nwaves, int nseg,
-                                                       float *partial) {
-    const int wg = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+                                                       int rows4, float *partial) {
+    // wave-uniform by construction; readfirstlane keeps the decode below and
+    // the per-view loads it feeds in scalar registers
+    const int wg = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (wg >= nwaves * nseg) return;  // a whole wavefront
     const int seg = wg / nwaves, w = wg - seg * nwaves;
-    const int ub = w % nub, t = w / nub;
-    const int j = t % a.nviews, vi = t / a.nviews;
+    int ub, j, vi;
+    if (rows4) {
+        const int q = w >> 2, t = q / nub;
+        ub = q % nub;
+        j = t % a.nviews;
+        vi = (t / a.nviews) * 4 + (w & 3);
+        if (vi >= a.nrows) return;  // a whole wavefront
+    } else {
+        const int t = w / nub;
+        ub = w % nub;
+        j = t % a.nviews;
+        vi = t / a.nviews;
+    }
     const int uo = ub * 64 + (threadIdx.x & 63);
     const bool inside = uo < a.nu_out;
     const size_t oidx = ((size_t)vi * a.nviews + j) * a.nu_out + (inside ? uo : 0);
@@ -617,7 +634,8 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         const int64_t planes = g.cone ? (int64_t)g.img_dim_z - 1 : (int64_t)g.img_dim_z;
         const int64_t bytes = nc * nc * planes * (g.cone ? 32 : 16);
         const int64_t nub = (a.nu_out + 63) / 64;
-        const int64_t nwaves = nub * a.nviews * a.nrows;
+        const int rows4 = a.nrows >= 4;
+        const int64_t nwaves = nub * a.nviews * (rows4 ? 4 * (((int64_t)a.nrows + 3) / 4) : a.nrows);
         // Small problems (e.g. one fan slice) would not fill the device with
         // one wavefront per 64 rays: split each ray's steps over nseg
         // wavefronts (about 32 wavefronts per CU in all) and add the partial
@@ -645,13 +663,13 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0,
                                    ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
-                                   (int)nseg, ctx->fp_partial);
+                                   (int)nseg, rows4, ctx->fp_partial);
             } else {
                 hipLaunchKernelGGL(fp_pack_kernel<false>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0,
                                    ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
-                                   (int)nseg, ctx->fp_partial);
+                                   (int)nseg, rows4, ctx->fp_partial);
             }
             if (nseg > 1)
                 hipLaunchKernelGGL(fp_segments_kernel, dim3(mct::blocks_for(nout, 256)), dim3(256), 0,
