@@ -288,17 +288,29 @@ def served_stream(n_jobs: int, qps: float) -> dict:
     plain = [j.fresh() for j in jobs]
     warm = [j.fresh() for j in jobs[:6]]
     burst = [replace(j.fresh(), arrival=0.0) for j in jobs]
+    serve.run_plain([replace(j.fresh(), arrival=0.0) for j in jobs[:6]])  # warm-up, untimed
+    sp = serve.run_plain(plain)
+
+    def same_as_plain(run):
+        # bit-identical to the plain path; then the pinned result buffers go
+        # back to the host allocator's cache, as a consumer that has written
+        # its results out would release them
+        ok = all(a.result is not None and b.result is not None and bool((a.result == b.result).all())
+                 for a, b in zip(run, plain))
+        for j in run:
+            j.result = None
+        return ok
+
     srv = serve.Server(hbm_budget=96 << 30)
     try:
         srv.run(warm)  # compile / allocate once
+        same_as_plain(warm)
         st = srv.run(jobs)
+        same = same_as_plain(jobs)
         sat = srv.run(burst)
+        same = same_as_plain(burst) and same
     finally:
         srv.close()
-    sp = serve.run_plain(plain)
-    same = all(a.result is not None and b.result is not None and bool((a.result == b.result).all())
-               for a, b in zip(jobs, plain))
-    same = same and all(bool((a.result == b.result).all()) for a, b in zip(burst, plain))
     mix = {}
     for j in jobs:
         mix[j.kind] = mix.get(j.kind, 0) + 1

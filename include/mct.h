@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 3
+#define MCT_ABI_VERSION 4
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -258,6 +258,12 @@ int mct_set_fp_kernel(mct_ctx *ctx, int32_t kind);
 /* Kernel the last mct_forward_project call on ctx ran (MCT_FP_DIRECT or
  * MCT_FP_CELLS; 0 none yet; -1 for a NULL ctx). */
 int mct_fp_last_kernel(const mct_ctx *ctx);
+/* Cell projector: when a launch has fewer wavefronts than waves_per_cu per CU,
+ * each ray's steps are split over up to 16 wavefronts (partial sums added in a
+ * fixed order by a second pass) to reach it.  0 restores the default (256).
+ * The split changes the order of summation: results with different settings
+ * agree to rounding, results with the same setting are identical. */
+int mct_set_fp_split(mct_ctx *ctx, int32_t waves_per_cu);
 
 /* Mgfbp_ir.GenEffectiveMapForwardProjectionAgent
  * run_mgfbp_iterative_recon_IRN_CGLS.py:679-833: 1 for rays whose samples
@@ -271,6 +277,33 @@ int mct_ray_mask(mct_plan *plan, float *mask);
  * counter-based generator (Philox4x32-10, Box-Muller) keyed by seed, so the
  * result depends only on (seed, element index).  In place, DEVICE. */
 int mct_add_noise(mct_ctx *ctx, float *sgm, int64_t n, float photon_number, uint64_t seed);
+
+/* ---- helical rebinning ------------------------------------------------------ */
+
+/* Scalars of Mgfbp_helical.SinogramRebinning run_mgfbp_helical.py:165-198
+ * (FP32, as the reference passes them). */
+typedef struct mct_helical {
+    int32_t n_u;                /* DetectorElementCountHorizontal */
+    int32_t n_rows;             /* detector rows of the helical sinogram */
+    int32_t n_views;            /* views of the helical sinogram */
+    int32_t view_count;         /* views of each rebinned fan sinogram */
+    float det_offset_vertical;  /* DetectorOffsetVertical [mm] */
+    float det_h;                /* DetectorElementHeight [mm] */
+    float sdd;                  /* SourceDetectorDistance [mm] */
+    float mag;                  /* sdd / sid */
+    float dis_per_view;         /* table feed per view [mm] */
+} mct_helical;
+
+/* Mgfbp_helical.SinogramRebinning for nz slice positions in ONE launch (the
+ * reference runs one launch per slice, run_mgfbp_helical.py:121-126):
+ * out[z][i][j] = linear interpolation between the detector rows around
+ * v_pos = (z_pos[z] - k dz) mag - offset in view k = floor(k0 - view_count/2
+ * + 1/2) + i, k0 = (z_pos[z] - offset/mag)/dz, scaled by
+ * sdd/sqrt(sdd^2 + v_pos^2); 0 where k < 0 or k > n_views - 2.
+ * sgm DEVICE FP32 [n_rows][n_views][n_u]; z_pos HOST [nz];
+ * out DEVICE FP32 [nz][view_count][n_u]. */
+int mct_helical_rebin(mct_ctx *ctx, const mct_helical *h, const float *sgm, const float *z_pos,
+                      int32_t nz, float *out);
 
 /* ---- iterative reconstruction --------------------------------------------- */
 

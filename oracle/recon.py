@@ -356,6 +356,47 @@ def backproject_ir(sgm, g, ray_mask=None, dtype=np.float32):
     return np.where(flag, acc, T(0)).astype(dtype)
 
 
+# ------------------------------------------------------- helical rebinning
+def helical_rebin(sgm, z_pos, view_count, off_v, det_h, sdd, mag, dz, dtype=np.float32):
+    """Mgfbp_helical.SinogramRebinning (run_mgfbp_helical.py:165-198) for
+    every slice position: sgm (rows, views, n_u) -> (len(z_pos), view_count,
+    n_u).  Slice z takes views k = floor(k0 - view_count/2 + 1/2) + i with
+    k0 = (z - off_v/mag)/dz; in view k it interpolates linearly between the
+    detector rows around v_pos = (z - k dz) mag - off_v (row index
+    -v_pos/det_h + (rows-1)/2, clamped to the first / last row) and scales by
+    sdd / sqrt(sdd^2 + v_pos^2); views outside [0, views-2] give 0.  The
+    view window is an index decision of the reference's FP32 kernel and is
+    taken in FP32 in both modes (a slice position on a tie would otherwise
+    pick another window than the kernel and the slice's image rotation)."""
+    T = _t(dtype)
+    F = np.float32
+    p = np.asarray(sgm, dtype=dtype)
+    rows, views, nu = p.shape
+    off_v, det_h, sdd, mag, dz = T(off_v), T(det_h), T(sdd), T(mag), T(dz)
+    out = np.zeros((len(z_pos), view_count, nu), dtype=dtype)
+    for zi, zf in enumerate(np.asarray(z_pos, dtype=dtype)):
+        k0 = (F(zf) - F(off_v) / F(mag)) / F(dz)
+        begin = T(np.floor(k0 - F(0.5) * F(view_count) + F(0.5)))
+        for i in range(view_count):
+            view = begin + T(i)
+            if not (0 <= view <= views - 2):
+                continue
+            k = int(view)
+            v_pos = (zf - view * dz) * mag - off_v
+            v_idx = (-v_pos) / det_h + (T(rows) - T(1)) / T(2)
+            if v_idx >= rows - 1:
+                val = p[rows - 1, k]
+            elif v_idx <= 0:
+                val = p[0, k]
+            else:
+                fl = np.floor(v_idx)
+                w = v_idx - fl
+                r = int(fl)
+                val = p[r, k] * (T(1) - w) + p[r + 1, k] * w
+            out[zi, i] = val * (sdd / np.sqrt(sdd * sdd + v_pos * v_pos))
+    return out
+
+
 # ------------------------------------------------------- forward projection
 def fp_scalars(g, count_mode=0, dtype=np.float32):
     """Scalars of run_mgfpj.py:486-527 (count_mode 1: run_mgfpj_v3.py:252)."""

@@ -24,7 +24,7 @@ MCT_FILTER_F32_DIRECT, MCT_FILTER_BF16_MFMA, MCT_FILTER_BF16X3_MFMA = 0, 1, 2
 MCT_BP_AUTO, MCT_BP_DIRECT, MCT_BP_LDS = 0, 1, 2
 MCT_FP_AUTO, MCT_FP_DIRECT, MCT_FP_CELLS = 0, 1, 2
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class MctError(RuntimeError):
@@ -54,6 +54,14 @@ class Geometry(C.Structure):
         ("center_z", C.POINTER(C.c_float)), ("matrix_a", C.POINTER(C.c_float)),
         ("x_s", C.POINTER(C.c_float)),
     ]
+
+
+class Helical(C.Structure):
+    """Mirror of struct mct_helical (include/mct.h)."""
+
+    _fields_ = [("n_u", C.c_int32), ("n_rows", C.c_int32), ("n_views", C.c_int32),
+                ("view_count", C.c_int32), ("det_offset_vertical", C.c_float), ("det_h", C.c_float),
+                ("sdd", C.c_float), ("mag", C.c_float), ("dis_per_view", C.c_float)]
 
 
 class BpJob(C.Structure):
@@ -87,6 +95,7 @@ SIGNATURES = {
     "mct_bp_last_kernel": (C.c_int, [_P]),
     "mct_set_fp_kernel": (C.c_int, [_P, _I32]),
     "mct_fp_last_kernel": (C.c_int, [_P]),
+    "mct_set_fp_split": (C.c_int, [_P, _I32]),
     "mct_backproject_batch": (C.c_int, [_P, C.POINTER(BpJob), _I32]),
     "mct_bp_batch_create": (C.c_int, [_P, C.POINTER(BpJob), _I32, C.POINTER(_P)]),
     "mct_bp_batch_launch": (C.c_int, [_P]),
@@ -94,6 +103,7 @@ SIGNATURES = {
     "mct_forward_project": (C.c_int, [_P, _P, _P, _P, _I32]),
     "mct_ray_mask": (C.c_int, [_P, _P]),
     "mct_add_noise": (C.c_int, [_P, _P, _I64, _F32, _U64]),
+    "mct_helical_rebin": (C.c_int, [_P, C.POINTER(Helical), _P, _P, _I32, _P]),
     "mct_backproject_ir": (C.c_int, [_P, _P, _P, _P]),
     "mct_dot_f64": (C.c_int, [_P, _P, _P, _I64, _P]),
     "mct_tv_weight": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _F64, _P]),

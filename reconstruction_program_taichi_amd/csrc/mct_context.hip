@@ -184,6 +184,7 @@ int mct_destroy(mct_ctx *ctx) {
     if (ctx->toep) hipFree(ctx->toep);
     if (ctx->taps_dev) hipFree(ctx->taps_dev);
     if (ctx->vkern_dev) hipFree(ctx->vkern_dev);
+    if (ctx->aux_dev) hipFree(ctx->aux_dev);
     if (ctx->fp_pack) hipFree(ctx->fp_pack);
     if (ctx->fp_partial) hipFree(ctx->fp_partial);
     if (ctx->batch_host) hipHostFree(ctx->batch_host);
@@ -235,6 +236,14 @@ int mct_set_fp_kernel(mct_ctx *ctx, int32_t kind) {
 }
 
 int mct_fp_last_kernel(const mct_ctx *ctx) { return ctx ? ctx->fp_last : -1; }
+
+int mct_set_fp_split(mct_ctx *ctx, int32_t waves_per_cu) {
+    if (!ctx) return mct::set_error(MCT_ERR_INVALID, "ctx is NULL");
+    if (waves_per_cu < 0 || waves_per_cu > 4096)
+        return mct::set_error(MCT_ERR_INVALID, "waves_per_cu must be in [0, 4096] (got %d)", waves_per_cu);
+    ctx->fp_split = waves_per_cu;
+    return MCT_OK;
+}
 
 int mct_plan_create(mct_ctx *ctx, const mct_geometry *g, mct_plan **out) {
     if (!out) return mct::set_error(MCT_ERR_INVALID, "out is NULL");

@@ -637,12 +637,14 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         const int rows4 = a.nrows >= 4;
         const int64_t nwaves = nub * a.nviews * (rows4 ? 4 * (((int64_t)a.nrows + 3) / 4) : a.nrows);
         // Small problems (e.g. one fan slice) would not fill the device with
-        // one wavefront per 64 rays: split each ray's steps over nseg
-        // wavefronts (about 32 wavefronts per CU in all) and add the partial
-        // sums in a second pass.
-        const int64_t target = (int64_t)ctx->cu_count * 32;
+        // one wavefront per 64 rays: split each ray's steps over nseg <= 16
+        // wavefronts (toward 256 wavefronts per CU in all, so the last round
+        // of wavefronts is a small part of the launch) and add the partial
+        // sums in a second pass.  (512^2 fan slice, 240 views: 0.46 ms with
+        // 32 wavefronts per CU, 0.35 ms with 256.)
+        const int64_t target = (int64_t)ctx->cu_count * (ctx->fp_split ? ctx->fp_split : 256);
         int64_t nseg = (target + nwaves - 1) / nwaves;
-        nseg = nseg < 1 ? 1 : (nseg > 8 ? 8 : nseg);
+        nseg = nseg < 1 ? 1 : (nseg > 16 ? 16 : nseg);
         const int64_t nblocks = (nwaves * nseg + 3) / 4;
         // 32-bit record offsets; gridDim.z of the pack launch; wavefront index
         const bool fits = nc >= 1 && planes >= 1 && bytes < ((int64_t)1 << 32) && planes <= 65535 &&

@@ -18,6 +18,7 @@ struct mct_ctx {
     int bp_last = 0;   // kernel of the last mct_backproject_fbp (mct_bp_last_kernel)
     int fp_kernel = 0; // forward projector: 0 automatic, 1 direct taps, 2 packed cells
     int fp_last = 0;   // kernel of the last mct_forward_project (mct_fp_last_kernel)
+    int fp_split = 0;  // cell projector: target wavefronts per CU when splitting rays (0: default)
     // packed volume of the forward projector (two orders of bilinear cells)
     float *fp_pack = nullptr;
     int64_t fp_pack_cap = 0;  // floats
@@ -38,6 +39,9 @@ struct mct_ctx {
     float *vkern_dev = nullptr;  // device copy of the vertical Gaussian taps
     int64_t vkern_cap = 0;
     uint64_t vkern_hash = 0;
+    float *aux_dev = nullptr;  // device copy of a small host table (helical slice positions)
+    int64_t aux_cap = 0;
+    uint64_t aux_hash = 0;
     // job table of mct_backproject_batch: pinned staging, device copy, and the
     // event that marks the staging buffer free again
     void *batch_host = nullptr, *batch_dev = nullptr;

@@ -80,6 +80,11 @@ class Context:
         scratch)."""
         call("mct_set_fp_kernel", self.h, self.FP_KERNELS[kind])
 
+    def set_fp_split(self, waves_per_cu: int = 0) -> None:
+        """Cell projector: split each ray's steps over several wavefronts when a
+        launch has fewer than waves_per_cu wavefronts per CU (0: default)."""
+        call("mct_set_fp_split", self.h, int(waves_per_cu))
+
     def fp_last_kernel(self) -> str:
         k = _abi.lib().mct_fp_last_kernel(self.h)
         return {0: "none", 1: "direct", 2: "cells"}.get(k, str(k))
@@ -159,6 +164,21 @@ class Context:
         h = C.c_void_p()
         call("mct_bp_batch_create", self.h, self._job_array(jobs), len(jobs), C.byref(h))
         return BpBatch(self, h, jobs)
+
+    def helical_rebin(self, sgm: torch.Tensor, z_pos, view_count: int, det_offset_vertical: float,
+                      det_h: float, sdd: float, mag: float, dis_per_view: float,
+                      out: torch.Tensor | None = None) -> torch.Tensor:
+        """Fan sinograms (len(z_pos), view_count, n_u) rebinned from a helical
+        cone-beam sinogram (rows, views, n_u) for every slice position."""
+        rows, views, nu = sgm.shape
+        z, zp = _host_f32(z_pos)
+        if out is None:
+            out = torch.empty((z.shape[0], view_count, nu), dtype=torch.float32, device=sgm.device)
+        if tuple(out.shape) != (z.shape[0], view_count, nu):
+            raise ValueError(f"out has shape {tuple(out.shape)}, expected {(z.shape[0], view_count, nu)}")
+        h = _abi.Helical(nu, rows, views, int(view_count), det_offset_vertical, det_h, sdd, mag, dis_per_view)
+        call("mct_helical_rebin", self.h, C.byref(h), _f32(sgm), zp, int(z.shape[0]), _f32(out))
+        return out
 
     def add_noise(self, sgm: torch.Tensor, photons: float, seed: int) -> torch.Tensor:
         call("mct_add_noise", self.h, _f32(sgm), sgm.numel(), float(photons), int(seed) & (2**64 - 1))

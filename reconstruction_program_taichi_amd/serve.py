@@ -9,8 +9,9 @@ iterate state on the device and overlaps the work of different requests:
   * a job is admitted when its buffers fit the HBM budget, then its sinogram
     is copied host -> device on the copy stream while earlier jobs compute;
   * every scheduling step collects the FBP/FDK jobs whose sinogram has
-    arrived, weights / filters / packs each, and back-projects all of them in
-    ONE persistent launch (mct_backproject_batch), on compute stream A;
+    arrived, weights / filters / packs each, back-projects each cone volume
+    in its own launch (its tiles fill the device) and all fan slices in ONE
+    persistent launch (mct_backproject_batch), on compute stream A;
   * CGLS-TV jobs run their iterations on a small pool of compute streams
     (round robin), overlapping A and each other;
   * results go device -> host on the copy stream; buffers return to the
@@ -252,7 +253,13 @@ class Server:
                 packed, layout = fbp_stages(self.ctx_a, plan, job, sgm)
                 vol = torch.empty(job.vol_shape(), dtype=torch.float32, device=self.dev)
                 job._dev.update(packed=packed, vol=vol)
-                by_dtype.setdefault(job.storage, []).append((plan, packed, job.storage, layout, vol))
+                if job.spec.cone:
+                    # a cone volume's tiles fill the device on their own; its
+                    # single launch runs the LDS kernel without the batch
+                    # kernel's per-item job lookup and register budget
+                    plan.backproject_fbp(packed, job.storage, layout, vol)
+                else:
+                    by_dtype.setdefault(job.storage, []).append((plan, packed, job.storage, layout, vol))
             for group in by_dtype.values():
                 for i in range(0, len(group), self.max_batch):
                     self.ctx_a.backproject_batch(group[i:i + self.max_batch])
