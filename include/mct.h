@@ -182,8 +182,8 @@ typedef struct mct_bp_job {
  * reference back-projects one file per kernel call, run_mgfbp.py:58-81).
  * Each job's result equals mct_backproject_fbp's for that job alone.  All
  * jobs share one dtype; plans must come from ctx.  The job table is copied
- * to the device through a context-owned pinned buffer, so calls on one
- * context are serialised on the host by that copy. */
+ * to the device through one of eight context-owned pinned staging slots; the
+ * call waits on the host only when all eight still hold batches in flight. */
 int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs);
 
 /* The same batched back-projection, prepared once and launched many times:

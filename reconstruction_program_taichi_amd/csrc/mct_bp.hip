@@ -1377,32 +1377,54 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     const int dtype = jobs[0].dtype;
     const size_t args_bytes = sizeof(BpArgs) * (size_t)njobs;
     const size_t bytes = args_bytes + sizeof(int) * ((size_t)njobs + 1);
-    if (ctx->batch_cap < (int64_t)bytes) {
-        if (ctx->batch_host) {
-            MCT_HIP(hipStreamSynchronize(ctx->stream));
-            hipHostFree(ctx->batch_host);
-            hipFree(ctx->batch_dev);
-            ctx->batch_host = ctx->batch_dev = nullptr;
-            ctx->batch_cap = 0;
+    // A staging slot (pinned table + device copy) whose previous batch has
+    // finished: its event follows that batch's kernel.  With every slot still
+    // in flight, wait for the oldest; the host does not otherwise wait for the
+    // stream, so a scheduler can queue other work behind a long batch.
+    int slot = -1;
+    for (int i = 0; i < mct::kBatchSlots && slot < 0; ++i) {
+        const int k = (ctx->batch_next + i) % mct::kBatchSlots;
+        mct::BatchSlot &b = ctx->batch_slot[k];
+        if (!b.ev) {
+            slot = k;
+            break;
         }
-        size_t cap = bytes < 65536 ? 65536 : bytes;
-        MCT_HIP(hipHostMalloc(&ctx->batch_host, cap, hipHostMallocDefault));
-        MCT_HIP(hipMalloc(&ctx->batch_dev, cap));
-        ctx->batch_cap = (int64_t)cap;
+        const hipError_t q = hipEventQuery(b.ev);
+        if (q == hipSuccess)
+            slot = k;
+        else if (q != hipErrorNotReady)
+            return mct::set_error(MCT_ERR_DEVICE, "batch staging event: %s", hipGetErrorString(q));
     }
-    if (!ctx->batch_ev) MCT_HIP(hipEventCreateWithFlags(&ctx->batch_ev, hipEventDisableTiming));
-    // the previous batch's table copy must have left the staging buffer
-    MCT_HIP(hipEventSynchronize(ctx->batch_ev));
-    BpArgs *args = reinterpret_cast<BpArgs *>(ctx->batch_host);
-    int *offs = reinterpret_cast<int *>(reinterpret_cast<char *>(ctx->batch_host) + args_bytes);
+    if (slot < 0) {
+        slot = ctx->batch_next;
+        MCT_HIP(hipEventSynchronize(ctx->batch_slot[slot].ev));
+    }
+    ctx->batch_next = (slot + 1) % mct::kBatchSlots;
+    mct::BatchSlot &b = ctx->batch_slot[slot];
+    if (!b.ev) MCT_HIP(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
+    if (b.cap < (int64_t)bytes) {  // the slot's last batch has finished: free to replace
+        if (b.host) hipHostFree(b.host);
+        if (b.dev) hipFree(b.dev);
+        b.host = b.dev = nullptr;
+        b.cap = 0;
+        const size_t cap = bytes < 65536 ? 65536 : bytes;
+        MCT_HIP(hipHostMalloc(&b.host, cap, hipHostMallocDefault));
+        MCT_HIP(hipMalloc(&b.dev, cap));
+        b.cap = (int64_t)cap;
+    }
+    BpArgs *args = reinterpret_cast<BpArgs *>(b.host);
+    int *offs = reinterpret_cast<int *>(reinterpret_cast<char *>(b.host) + args_bytes);
     int total = 0, lds_cells = 0;
     int rc = fill_batch(ctx, jobs, njobs, dtype, args, offs, total, lds_cells);
     if (rc) return rc;
-    MCT_HIP(hipMemcpyAsync(ctx->batch_dev, ctx->batch_host, bytes, hipMemcpyHostToDevice, ctx->stream));
-    MCT_HIP(hipEventRecord(ctx->batch_ev, ctx->stream));
-    const BpArgs *dargs = reinterpret_cast<const BpArgs *>(ctx->batch_dev);
-    const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(ctx->batch_dev) + args_bytes);
-    return launch_batch(ctx, dargs, doffs, njobs, total, dtype, lds_cells);
+    MCT_HIP(hipMemcpyAsync(b.dev, b.host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    const BpArgs *dargs = reinterpret_cast<const BpArgs *>(b.dev);
+    const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(b.dev) + args_bytes);
+    rc = launch_batch(ctx, dargs, doffs, njobs, total, dtype, lds_cells);
+    const hipError_t e = hipEventRecord(b.ev, ctx->stream);  // after the kernel: slot reusable
+    if (rc) return rc;
+    if (e != hipSuccess) return mct::set_error(MCT_ERR_DEVICE, "batch staging event: %s", hipGetErrorString(e));
+    return MCT_OK;
 }
 
 int mct_bp_batch_create(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs, mct_bp_batch **out) {

@@ -187,9 +187,12 @@ int mct_destroy(mct_ctx *ctx) {
     if (ctx->aux_dev) hipFree(ctx->aux_dev);
     if (ctx->fp_pack) hipFree(ctx->fp_pack);
     if (ctx->fp_partial) hipFree(ctx->fp_partial);
-    if (ctx->batch_host) hipHostFree(ctx->batch_host);
-    if (ctx->batch_dev) hipFree(ctx->batch_dev);
-    if (ctx->batch_ev) hipEventDestroy(ctx->batch_ev);
+    for (mct::BatchSlot &b : ctx->batch_slot) {
+        if (b.ev) hipEventSynchronize(b.ev);
+        if (b.host) hipHostFree(b.host);
+        if (b.dev) hipFree(b.dev);
+        if (b.ev) hipEventDestroy(b.ev);
+    }
     delete ctx;
     return MCT_OK;
 }

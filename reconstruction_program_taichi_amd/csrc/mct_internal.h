@@ -7,6 +7,18 @@
 
 #include "../../include/mct.h"
 
+namespace mct {
+// A job-table staging slot of mct_backproject_batch: pinned host table, its
+// device copy, and an event recorded after the batch's kernel (the slot is
+// free again once it has completed).
+struct BatchSlot {
+    void *host = nullptr, *dev = nullptr;
+    int64_t cap = 0;
+    hipEvent_t ev = nullptr;
+};
+constexpr int kBatchSlots = 8;
+}  // namespace mct
+
 struct mct_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -42,11 +54,9 @@ struct mct_ctx {
     float *aux_dev = nullptr;  // device copy of a small host table (helical slice positions)
     int64_t aux_cap = 0;
     uint64_t aux_hash = 0;
-    // job table of mct_backproject_batch: pinned staging, device copy, and the
-    // event that marks the staging buffer free again
-    void *batch_host = nullptr, *batch_dev = nullptr;
-    int64_t batch_cap = 0;
-    hipEvent_t batch_ev = nullptr;
+    // job tables of mct_backproject_batch: a ring of staging slots
+    mct::BatchSlot batch_slot[mct::kBatchSlots];
+    int batch_next = 0;
 };
 
 struct mct_plan {
