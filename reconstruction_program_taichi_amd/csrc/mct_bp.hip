@@ -548,13 +548,6 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
         const float zfirst = ((float)z0 - halfz) * a.vh + a.cz;
         const float zstep_t = a.vh * a.vinv;
         const uint32_t row_stride = (uint32_t)pitch * (uint32_t)sizeof(Raw);  // bytes per window row
-        // The window of a (tile, chunk, view) bounds the row coordinate of
-        // every voxel of the tile and chunk (1/L is extremal at the tile's
-        // corners, z at the chunk's first and last slice), widened by one
-        // row.  Where it is not cut by the detector's first or last row and
-        // the chunk has all its slices, every row coordinate therefore lies
-        // inside it and the clamp can be skipped.
-        const bool full_chunk = nk == kLdsZC;
 
         load_tables(0);
         BpWin wc = wt[0];
@@ -587,39 +580,21 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
                 const unsigned char *src =
                     smem + (uint32_t)((j & 1) * cells + lc) * (uint32_t)sizeof(Raw);
                 // reads in batches of 8, all in flight before the first is used
-                if (full_chunk && wc.r0 > 0 && wc.r0 + wc.h < a.nv - 1) {
 #pragma unroll
-                    for (int k0 = 0; k0 < kLdsZC; k0 += 8) {
-                        Raw q[8];
-                        float rv[8];
+                for (int k0 = 0; k0 < kLdsZC; k0 += 8) {
+                    Raw q[8];
+                    float rv[8];
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            const float tk = fmaf((float)(k0 + i), dt, t0);
-                            const uint32_t vf = (uint32_t)tk;
-                            rv[i] = __builtin_amdgcn_fractf(tk);
-                            q[i] = *reinterpret_cast<const Raw *>(src + __umul24(vf, row_stride));
-                        }
-#pragma unroll
-                        for (int i = 0; i < 8; ++i)
-                            acc[k0 + i] = fmaf(cv.coef, bilerp(T::cvt(q[i]), cv.ru, rv[i]), acc[k0 + i]);
+                    for (int i = 0; i < 8; ++i) {
+                        float tk = fmaf((float)(k0 + i), dt, t0);
+                        tk = __builtin_amdgcn_fmed3f(tk, 0.f, tlim);
+                        const uint32_t vf = (uint32_t)tk;
+                        rv[i] = __builtin_amdgcn_fractf(tk);
+                        q[i] = *reinterpret_cast<const Raw *>(src + __umul24(vf, row_stride));
                     }
-                } else {
 #pragma unroll
-                    for (int k0 = 0; k0 < kLdsZC; k0 += 8) {
-                        Raw q[8];
-                        float rv[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            float tk = fmaf((float)(k0 + i), dt, t0);
-                            tk = __builtin_amdgcn_fmed3f(tk, 0.f, tlim);
-                            const uint32_t vf = (uint32_t)tk;
-                            rv[i] = __builtin_amdgcn_fractf(tk);
-                            q[i] = *reinterpret_cast<const Raw *>(src + __umul24(vf, row_stride));
-                        }
-#pragma unroll
-                        for (int i = 0; i < 8; ++i)
-                            acc[k0 + i] = fmaf(cv.coef, bilerp(T::cvt(q[i]), cv.ru, rv[i]), acc[k0 + i]);
-                    }
+                    for (int i = 0; i < 8; ++i)
+                        acc[k0 + i] = fmaf(cv.coef, bilerp(T::cvt(q[i]), cv.ru, rv[i]), acc[k0 + i]);
                 }
             }
             if (more) put((j + 1) & 1, wn);
