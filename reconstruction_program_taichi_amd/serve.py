@@ -8,7 +8,8 @@ iterate state on the device and overlaps the work of different requests:
 
   * a job is admitted when its buffers fit the HBM budget, then its sinogram
     is copied host -> device on the copy stream while earlier jobs compute;
-    uploads go in arrival order (h2d_window can bound the bytes in flight);
+    uploads go in arrival order (small_bytes can let small uploads go
+    first, h2d_window can bound the bytes in flight);
   * every scheduling step collects the FBP/FDK jobs whose sinogram has
     arrived, weights / filters / packs each, back-projects each cone volume
     in its own launch (its tiles fill the device) and all fan slices in ONE
@@ -198,7 +199,7 @@ class Server:
     """Continuous-batching server on one device (see the module comment)."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, max_batch: int = 64,
-                 cgls_streams: int = 1, h2d_window: int = 1 << 40):
+                 cgls_streams: int = 1, h2d_window: int = 1 << 40, small_bytes: int = 0):
         self.device = device
         self.dev = torch.device("cuda", device)
         self.s_copy = torch.cuda.Stream(self.dev)  # host <-> device copies
@@ -220,6 +221,10 @@ class Server:
         self.used = 0
         self.max_batch = max_batch
         self.h2d_window = int(h2d_window)  # bytes of sinogram uploads in flight
+        # uploads up to this size may go ahead of larger ones (0: arrival
+        # order, which measured best on the mixed burst: 210 jobs/s against
+        # 190-200 with 16-64 MB)
+        self.small_bytes = int(small_bytes)
         self.rejected: list[Job] = []
 
     def _admit(self, job: Job, now: float) -> bool:
@@ -305,8 +310,16 @@ class Server:
         t0 = time.perf_counter()
         while order or pending or loading or running:
             now = time.perf_counter() - t0
+            arrived = False
             while order and order[0].arrival <= now:
                 pending.append(order.popleft())
+                arrived = True
+            if arrived and self.small_bytes > 0 and len(pending) > 1:
+                # small uploads (fan slices, CGLS jobs) go ahead of large ones,
+                # arrival order within each class: they cost the large jobs
+                # next to nothing and start the device (and the CGLS stream)
+                # while the large sinograms are in flight
+                pending = deque(sorted(pending, key=lambda j: j.sgm_host.numel() * 4 > self.small_bytes))
             # Uploads are issued in arrival order while less than h2d_window
             # bytes are in flight (unbounded by default: bounding it delayed
             # the large jobs more than it helped the small ones).
