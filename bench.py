@@ -220,7 +220,10 @@ def main() -> int:
                               "the taps are served by LDS/L1/L2, so frac > 1 is expected",
                      "gather": gather_roofline(kernel, U * 4 * tap_bytes, bp_ms)},
     }
-    line["config"]["pcie_inclusive"] = pcie_inclusive(raw, work, vol, fbp)
+    raw_h = torch.empty(raw.shape, dtype=raw.dtype, pin_memory=True)
+    raw_h.copy_(raw)
+    line["config"]["pcie_inclusive"] = pcie_inclusive(raw_h, work, vol, fbp)
+    line["config"]["streamed"] = streamed_inclusive(raw_h, vol, args.storage, args.filter)
     if not args.no_cpu:
         filt_host = fbp.last_filtered.cpu().numpy()
         cb = cpu_baseline(fbp, filt_host, args.cpu_slices)
@@ -245,14 +248,13 @@ def main() -> int:
     return 0
 
 
-def pcie_inclusive(raw, work, vol, fbp, reps: int = 3) -> dict:
-    """The same FDK step with the sinogram coming from pinned host memory and
-    the volume going back to it (the rate a file-driven caller sees before
-    disk I/O); reported beside the resident-input value, never as it."""
+def pcie_inclusive(raw_h, work, vol, fbp, reps: int = 3) -> dict:
+    """The same FDK step with the sinogram coming from pinned host memory
+    (raw_h) and the volume going back to it (the rate a file-driven caller
+    sees before disk I/O); reported beside the resident-input value, never
+    as it."""
     import torch
 
-    raw_h = torch.empty(raw.shape, dtype=raw.dtype, pin_memory=True)
-    raw_h.copy_(raw)
     vol_h = torch.empty(vol.shape, dtype=vol.dtype, pin_memory=True)
     times = []
     for _ in range(reps):
@@ -264,10 +266,38 @@ def pcie_inclusive(raw, work, vol, fbp, reps: int = 3) -> dict:
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     ms = float(np.mean(times)) * 1e3
-    nx, nz, nv = vol.shape[2], vol.shape[0], raw.shape[1]
-    gb = (raw.numel() + vol.numel()) * 4 / 1e9
+    nx, nz, nv = vol.shape[2], vol.shape[0], raw_h.shape[1]
+    gb = (raw_h.numel() + vol.numel()) * 4 / 1e9
     return {"ms_per_step": round(ms, 3), "gups": round(float(nx) * nx * nz * nv / (ms * 1e-3) / 1e9, 2),
             "host_bytes_gb": round(gb, 3)}
+
+
+def streamed_inclusive(raw_h, vol, storage: str, filter_method: str, reps: int = 3) -> dict:
+    """The view-block streaming FBP (run_mgfbp_v2 drop-in, default block
+    size) on the same pinned host sinogram: the upload of block b+1 overlaps
+    the kernels of block b; the volume is copied back to pinned host memory.
+    vol holds the resident path's volume; the streamed one must equal it."""
+    import torch
+
+    from reconstruction_program_taichi_amd.fbp_v2 import Mgfbp_v2
+
+    f = Mgfbp_v2(c1_config(storage, filter_method))
+    out = torch.empty_like(vol)
+    vol_h = torch.empty(vol.shape, dtype=vol.dtype, pin_memory=True)
+    f.reconstruct_streamed(raw_h, out)  # plan tables
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        f.reconstruct_streamed(raw_h, out)
+        vol_h.copy_(out, non_blocking=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = float(np.mean(times)) * 1e3
+    nx, nz, nv = vol.shape[2], vol.shape[0], raw_h.shape[1]
+    return {"views_per_block": f.views_per_block, "ms_per_step": round(ms, 3),
+            "gups": round(float(nx) * nx * nz * nv / (ms * 1e-3) / 1e9, 2),
+            "equals_resident": bool(torch.equal(out, vol))}
 
 
 def served_stream(n_jobs: int, qps: float) -> dict:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 4
+#define MCT_ABI_VERSION 5
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -108,6 +108,16 @@ int mct_destroy(mct_ctx *ctx);
 /* hip_stream: a hipStream_t (NULL = null stream) */
 int mct_set_stream(mct_ctx *ctx, void *hip_stream);
 int mct_synchronize(mct_ctx *ctx);
+
+/* Views [view_begin, view_begin + view_count) of a HOST FP32 sinogram
+ * [n_v][n_views][n_u] into a DEVICE block [n_v][view_count][n_u]: one
+ * two-dimensional asynchronous copy on `stream` (a hipStream_t; NULL: the
+ * context's stream).  For the copy to overlap kernels, host_sgm must be
+ * pinned (hipHostMalloc) memory.  Replaces the per-view upload
+ * img_sgm_taichi.from_numpy(img_sgm[:, view_idx:view_idx+1, :]) of
+ * run_mgfbp_v2.py:71. */
+int mct_copy_views(mct_ctx *ctx, void *stream, const float *host_sgm, int32_t n_v, int32_t n_views,
+                   int32_t n_u, int32_t view_begin, int32_t view_count, float *dev_block);
 const char *mct_last_error(void);
 int mct_abi_version(void);
 
@@ -131,6 +141,14 @@ int mct_plan_destroy(mct_plan *plan);
  * weight.  In place on a DEVICE FP32 sinogram [n_v][n_views][n_u].
  * bh_coef: HOST array of bh_order floats (bh_order <= 16). */
 int mct_weight_sinogram(mct_plan *plan, float *sgm, const float *bh_coef, int32_t bh_order);
+
+/* The same weighting for a block of views, as the view-by-view FBP
+ * Mgfbp_v2.BHCorrection / Mgfbp_v2.WeightSgm run_mgfbp_v2.py:120-186 applies
+ * it: sgm is a DEVICE FP32 [n_v][view_count][n_u] block holding views
+ * [view_begin, view_begin + view_count) of the plan's scan; the Parker weight
+ * uses each view's angle in the whole scan. */
+int mct_weight_views(mct_plan *plan, float *sgm, int32_t view_begin, int32_t view_count,
+                     const float *bh_coef, int32_t bh_order);
 
 /* Vertical Gaussian of Mgfbp.ConvolveSgmAndKernel run_mgfbp.py:1019-1031:
  * out[i,j,k] = det_h * sum_{|i-n|<=10} in[n,j,k] * kern[i + rows-1 - n].
@@ -165,6 +183,21 @@ int64_t mct_packed_bytes(int32_t rows, int32_t views, int32_t n_u, int32_t dtype
  * vol: DEVICE FP32 [img_dim_z][img_dim][img_dim], overwritten. */
 int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
                         float *vol);
+
+/* View-by-view FBP, Mgfbp_v2.MainFunction run_mgfbp_v2.py:62-94 with
+ * Mgfbp_v2.BackProjectionPixelDriven :238-364 and SetTruncatedRegionToZero
+ * :232-236, for a block of views at a time: back-projects views
+ * [view_begin, view_begin + view_count) of the plan's scan from filt, a
+ * packed block holding only those views (as mct_pack_sinogram writes it for
+ * view_count views), and ADDS them to vol, voxel by voxel in view order, so a
+ * scan streamed block after block into a zeroed volume sums every voxel in
+ * the same order as mct_backproject_fbp.  A voxel that projects off the
+ * detector is set to -10000 (the reference's truncation mark; later blocks
+ * keep adding to it).  finish != 0 (the last block): every voxel below -5000
+ * is then set to 0.  The view weight dbeta uses the neighbouring views of
+ * the whole scan. */
+int mct_backproject_views(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
+                          int32_t view_begin, int32_t view_count, int32_t finish, float *vol);
 
 /* One job of a batched back-projection: a flat-detector axial plan, its
  * packed filtered sinogram (QUAD for cone beam, PAIR for fan beam) and the

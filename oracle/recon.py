@@ -204,11 +204,13 @@ def _pm_project(pm, j, x, y, z, T):
     return mag, uidx, vidx
 
 
-def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
-    """Mgfbp.BackProjectionPixelDriven (run_mgfbp.py:1062-1187), axial view,
-    with the P-matrix branch when g["pmatrix"] (12 per view) is given.
-    filt: (rows, views, n_u).  g: dict of geometry scalars and arrays
-    (angles, u, v).  Returns (nz, ny, nx)."""
+def _fbp_view_terms(filt, g, dtype):
+    """The view loop of Mgfbp.BackProjectionPixelDriven (run_mgfbp.py:1107-1187),
+    axial view, P-matrix branch when g["pmatrix"] (12 per view) is given: for
+    each view j in order, (ok, term) over the (nz, ny, nx) grid, ok = the voxel
+    passes view j's u (and, cone beam, v) range tests, term = its weighted
+    contribution sid/L^2 * value * dbeta_j * div_factor (meaningless where not
+    ok).  filt: (rows, views, n_u)."""
     T = _t(dtype)
     f = np.asarray(filt, dtype=dtype)
     nx, nz = g["img_dim"], g["img_dim_z"]
@@ -235,8 +237,6 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
     y = (-xa) * sr + ya * cr
     z = ((iz - halfz) * vh + cz)[:, None, None]
     x, y, z = np.broadcast_arrays(x, y, z)
-    acc = np.zeros(x.shape, dtype=dtype)
-    alive = np.ones(x.shape, dtype=bool)
     u0, du = u[0], u[1] - u[0]
     if cone:
         v0, dv = v[0], v[1] - v[0]
@@ -258,7 +258,7 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
             else:
                 uu = mag * yr
             uidx = (uu - u0) / du
-        alive &= ~((uidx < 0) | (uidx + T(1) > T(nu - 1)))
+        ok = ~((uidx < 0) | (uidx + T(1) > T(nu - 1)))
         uf = np.clip(np.floor(uidx), 0, nu - 2).astype(np.int64)
         ru = uidx - uf.astype(dtype)
         if curved:
@@ -269,7 +269,7 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
         if cone:
             vidx = vidx_pm if pm is not None else (mag * z - v0) / dh * adv / dv
             vff = np.floor(vidx)
-            alive &= ~((vff < 0) | (vff + T(1) > T(nrows - 1)))
+            ok &= ~((vff < 0) | (vff + T(1) > T(nrows - 1)))
             vf = np.clip(vff, 0, nrows - 2).astype(np.int64)
             rv = vidx - vf.astype(dtype)
             p0 = f[vf, j, uf] * (T(1) - ru) + f[vf, j, uf + 1] * ru
@@ -277,9 +277,41 @@ def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
             val = (T(1) - rv) * p0 + rv * p1
         else:
             val = (T(1) - ru) * f[slice_idx, j, uf] + ru * f[slice_idx, j, uf + 1]
-        acc = acc + (sid * w) * val * db[j] * div
+        yield ok, (sid * w) * val * db[j] * div
+
+
+def backproject_fbp(filt, g, dtype=np.float32, return_mask=False):
+    """Mgfbp.BackProjectionPixelDriven (run_mgfbp.py:1062-1187), axial view,
+    with the P-matrix branch when g["pmatrix"] (12 per view) is given: the
+    terms summed in view order; a voxel that fails a range test in any view
+    is 0.  filt: (rows, views, n_u).  g: dict of geometry scalars and arrays
+    (angles, u, v).  Returns (nz, ny, nx)."""
+    T = _t(dtype)
+    acc = alive = None
+    for ok, term in _fbp_view_terms(filt, g, dtype):
+        if acc is None:
+            acc = np.zeros(term.shape, dtype=dtype)
+            alive = np.ones(term.shape, dtype=bool)
+        alive &= ok
+        acc = acc + term
     out = np.where(alive, acc, T(0)).astype(dtype)
     return (out, alive) if return_mask else out
+
+
+def backproject_fbp_v2(filt, g, vol=None, dtype=np.float32):
+    """The view-by-view FBP of Mgfbp_v2 (run_mgfbp_v2.py:71-94): each view's
+    BackProjectionPixelDriven (:238-364) adds its term to the volume, a view
+    whose range test fails sets the voxel to -10000 (:312-313, 332-333) and
+    later views keep adding; SetTruncatedRegionToZero (:232-236) then sets
+    every voxel below -5000 to 0.  vol: the volume the views are added to
+    (zeros when None).  Returns (nz, ny, nx)."""
+    T = _t(dtype)
+    acc = None if vol is None else np.array(vol, dtype=dtype)
+    for ok, term in _fbp_view_terms(filt, g, dtype):
+        if acc is None:
+            acc = np.zeros(term.shape, dtype=dtype)
+        acc = np.where(ok, acc + term, T(-10000)).astype(dtype)
+    return np.where(acc < T(-5000), T(0), acc).astype(dtype)
 
 
 def backproject_ir(sgm, g, ray_mask=None, dtype=np.float32):

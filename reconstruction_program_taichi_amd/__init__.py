@@ -10,7 +10,8 @@ from .config import (PI, ReadConfigFile, XuLeastSquareSolution, imreadRaw, load_
 
 __all__ = ["PI", "ReadConfigFile", "XuLeastSquareSolution", "imreadRaw", "load_jsonc",
            "remove_comments", "save_jsonc", "run_mgfbp", "Mgfbp", "run_mgfpj", "Mgfpj",
-           "run_mgfpj_v2", "run_mgfpj_v3", "Mgfpj_v3", "run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"]
+           "run_mgfpj_v2", "run_mgfpj_v3", "Mgfpj_v3", "run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs",
+           "run_mgfbp_v2", "Mgfbp_v2", "run_mgfbp_helical", "Mgfbp_helical"]
 
 
 def __getattr__(name):
@@ -26,6 +27,12 @@ def __getattr__(name):
     if name in ("run_mgfpj_v3", "Mgfpj_v3"):
         from . import fpj_v3
         return getattr(fpj_v3, name)
+    if name in ("run_mgfbp_v2", "Mgfbp_v2"):
+        from . import fbp_v2
+        return getattr(fbp_v2, name)
+    if name in ("run_mgfbp_helical", "Mgfbp_helical"):
+        from . import fbp_helical
+        return getattr(fbp_helical, name)
     if name in ("run_mgfbp_ir", "Mgfbp_ir", "Mgfbp_ir_piccs"):
         from . import ir
         return getattr(ir, name)

@@ -64,7 +64,21 @@ struct BpArgs {
     const BpWin *win;
     const int2 *iv;
     int pitch, rows;
+    // FBP view block: views [j0, j0 + nvb) of the scan, read from a packed
+    // sinogram that holds only those views (j0 = 0, nvb = nviews: all).
+    // accum 0: the volume is overwritten (masked voxels 0); 1: the block's
+    // contributions are added to the volume in view order and masked voxels
+    // are marked -10000; 2: as 1, then every voxel below -5000 is set to 0.
+    int j0, nvb, accum;
 };
+
+// Stored value of a voxel whose running sum is acc (alive: it passed the range
+// tests of every view the launch decides on).
+__device__ __forceinline__ float bp_result(const BpArgs &a, bool alive, float acc) {
+    float v = alive ? acc : (a.accum ? -10000.f : 0.f);
+    if (a.accum == 2 && v < -5000.f) v = 0.f;
+    return v;
+}
 
 template <int DT> struct Taps;
 template <> struct Taps<MCT_F32> {
@@ -253,15 +267,18 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
     const float halfz = (float)(a.nz - 1) / 2.0f;
     const float zfirst = ((float)z0 - halfz) * a.vh + a.cz;
     const float zstep_t = a.vh * a.vinv;
+    const size_t plane = (size_t)a.nx * a.nx;
+    float *out = a.vol + (size_t)iy * a.nx + ix;
     float acc[ZC];
 #pragma unroll
-    for (int k = 0; k < ZC; ++k) acc[k] = 0.f;
+    for (int k = 0; k < ZC; ++k) acc[k] = (a.accum && k < nk) ? out[(size_t)(z0 + k) * plane] : 0.f;
     int lo = 0, hi = nk - 1;
     const uint32_t rowbytes = (uint32_t)(a.nu - 1) * Taps<DT>::kQuadBytes;
     const size_t viewbytes = (size_t)(a.nv - 1) * rowbytes;
     const char *sino = reinterpret_cast<const char *>(a.sino);
-    for (int j = 0; j < a.nviews; ++j) {
-        const float4 t = a.vt[j];
+    const float4 *vt = a.vt + a.j0;
+    for (int j = 0; j < a.nvb; ++j) {
+        const float4 t = vt[j];
         ColumnView cv = column_view(a, x, y, t);
         if (!cv.ok) {
             hi = -1;
@@ -296,11 +313,9 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
                 acc[k0 + i] = fmaf(cv.coef, bilerp(Taps<DT>::cvt(raw[i]), cv.ru, rv[i]), acc[k0 + i]);
         }
     }
-    const size_t plane = (size_t)a.nx * a.nx;
-    float *out = a.vol + (size_t)iy * a.nx + ix;
 #pragma unroll
     for (int k = 0; k < ZC; ++k)
-        if (k < nk) out[(size_t)(z0 + k) * plane] = (k >= lo && k <= hi) ? acc[k] : 0.f;
+        if (k < nk) out[(size_t)(z0 + k) * plane] = bp_result(a, k >= lo && k <= hi, acc[k]);
 }
 
 template <int ZC, int B, int DT>
@@ -466,8 +481,10 @@ __global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restri
 // One workgroup tile (tile, chunk) = block of job a.  a.pitch x a.rows: the
 // LDS window shape (largest w and h of the job's table); NQ: staging slots
 // per thread, NQ * 256 >= pitch * rows; smem: two buffers of pitch * rows
-// packed cells.  Every thread of the workgroup must call this (barriers).
-template <int DT, int NQ>
+// packed cells; ACC: the launch may be a view block (a.accum != 0), else the
+// volume is overwritten.  Every thread of the workgroup must call this
+// (barriers).
+template <int DT, int NQ, bool ACC>
 __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsigned char *smem) {
     using T = Taps<DT>;
     using Raw = typename T::Raw;
@@ -492,6 +509,13 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
     float acc[kLdsZC];
 #pragma unroll
     for (int k = 0; k < kLdsZC; ++k) acc[k] = 0.f;
+    if (ACC && a.accum && inside) {  // a view block adds to the volume
+        const size_t plane = (size_t)a.nx * a.nx;
+        const float *in = a.vol + (size_t)iy * a.nx + ix;
+#pragma unroll
+        for (int k = 0; k < kLdsZC; ++k)
+            if (k < nk) acc[k] = in[(size_t)(z0 + k) * plane];
+    }
 
     if (__syncthreads_or(alive)) {
         const int tid = threadIdx.x;
@@ -510,7 +534,10 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
             srow[q] = e < cells ? r : 0x7fffffff;
             soff[q] = (uint32_t)r * rowbytes + (uint32_t)c * T::kQuadBytes;
         }
-        const BpWin *wt = win + ((size_t)tile * a.nzc + zci) * a.nviews;
+        // (a whole-scan launch reads the fields as given: j0 = 0, nvb = nviews)
+        const BpWin *wt = win + ((size_t)tile * a.nzc + zci) * a.nviews + (ACC ? a.j0 : 0);
+        const float4 *vt = ACC ? a.vt + a.j0 : a.vt;
+        const int nvb = ACC ? a.nvb : a.nviews;
         Raw stage[NQ];
         auto fetch = [&](int j, const BpWin &w) {
             const size_t origin = (size_t)w.r0 * rowbytes + (size_t)w.c0 * T::kQuadBytes;
@@ -536,10 +563,10 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
         __shared__ BpWin s_win[64];
         auto load_tables = [&](int j0) {
             if (tid < 64) {
-                if (j0 + tid < a.nviews) s_vt[tid] = a.vt[j0 + tid];
+                if (j0 + tid < nvb) s_vt[tid] = vt[j0 + tid];
             } else if (tid < 128) {
                 const int jj = j0 + 1 + (tid - 64);
-                if (jj < a.nviews) s_win[tid - 64] = wt[jj];
+                if (jj < nvb) s_win[tid - 64] = wt[jj];
             }
         };
         float x, y;
@@ -554,12 +581,12 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
         fetch(0, wc);
         put(0, wc);
         __syncthreads();
-        for (int j = 0; j < a.nviews; ++j) {
+        for (int j = 0; j < nvb; ++j) {
             if (j && (j & 63) == 0) {  // every wave has finished the previous batch's entries
                 load_tables(j);
                 __syncthreads();
             }
-            const bool more = j + 1 < a.nviews;
+            const bool more = j + 1 < nvb;
             BpWin wn = wc;
             if (more) {
                 wn = s_win[j & 63];
@@ -606,16 +633,18 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
         const size_t plane = (size_t)a.nx * a.nx;
         float *out = a.vol + (size_t)iy * a.nx + ix;
 #pragma unroll
-        for (int k = 0; k < kLdsZC; ++k)
-            if (k < nk) out[(size_t)(z0 + k) * plane] = (k >= lo && k <= hi) ? acc[k] : 0.f;
+        for (int k = 0; k < kLdsZC; ++k) {
+            const bool ok = k >= lo && k <= hi;
+            if (k < nk) out[(size_t)(z0 + k) * plane] = ACC ? bp_result(a, ok, acc[k]) : (ok ? acc[k] : 0.f);
+        }
     }
 }
 
-template <int DT, int NQ>
+template <int DT, int NQ, bool ACC>
 __global__ __launch_bounds__(256) void bp_cone_lds_kernel(BpArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cone_lds_tile<DT, NQ>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
-                          smem);
+    cone_lds_tile<DT, NQ, ACC>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
+                               smem);
 }
 
 // Fan beam, axial view, PAIR taps; slice z reads detector row z.
@@ -628,14 +657,17 @@ __device__ __forceinline__ void fan_tile(const BpArgs &a, int block) {
     const int nk = a.nz - z0 < ZC ? a.nz - z0 : ZC;
     float x, y;
     voxel_xy(a, ix, iy, x, y);
+    const size_t plane = (size_t)a.nx * a.nx;
+    float *out = a.vol + (size_t)iy * a.nx + ix;
     float acc[ZC];
 #pragma unroll
-    for (int k = 0; k < ZC; ++k) acc[k] = 0.f;
+    for (int k = 0; k < ZC; ++k) acc[k] = (a.accum && k < nk) ? out[(size_t)(z0 + k) * plane] : 0.f;
     bool alive = true;
     const int ncol = a.nu - 1;
-    const size_t row_stride = (size_t)a.nviews * ncol;
-    for (int j = 0; j < a.nviews; ++j) {
-        const float4 t = a.vt[j];
+    const size_t row_stride = (size_t)a.nvb * ncol;
+    const float4 *vt = a.vt + a.j0;
+    for (int j = 0; j < a.nvb; ++j) {
+        const float4 t = vt[j];
         ColumnView cv = column_view(a, x, y, t);
         if (!cv.ok) {
             alive = false;
@@ -650,11 +682,9 @@ __device__ __forceinline__ void fan_tile(const BpArgs &a, int block) {
             }
         }
     }
-    const size_t plane = (size_t)a.nx * a.nx;
-    float *out = a.vol + (size_t)iy * a.nx + ix;
 #pragma unroll
     for (int k = 0; k < ZC; ++k)
-        if (k < nk) out[(size_t)(z0 + k) * plane] = alive ? acc[k] : 0.f;
+        if (k < nk) out[(size_t)(z0 + k) * plane] = bp_result(a, alive, acc[k]);
 }
 
 template <int ZC, int DT>
@@ -689,7 +719,7 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict_
         const int local = item - offsets[lo];
         if constexpr (NQ > 0) {
             if (a.cone && a.win) {
-                cone_lds_tile<DT, NQ>(a, local, smem);
+                cone_lds_tile<DT, NQ, false>(a, local, smem);
                 continue;
             }
         }
@@ -736,9 +766,11 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
     float x, y, z;
     voxel_xyz(a, ix, iy, iz, a.cz, x, y, z);
     const int ncol = a.nu - 1;
-    float acc = 0.f;
+    float acc = a.accum ? a.vol[id] : 0.f;
     bool alive = true;
-    for (int j = 0; j < a.nviews; ++j) {
+    // jl: view within the packed block; j = a.j0 + jl: view of the scan
+    for (int jl = 0; jl < a.nvb; ++jl) {
+        const int j = a.j0 + jl;
         const float4 t = a.vt[j];
         float L = a.sid - x * t.x - y * t.y;
         float mag, uidx;
@@ -783,11 +815,11 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
             int vf = (int)vff;
             float rv = vidx - vff;
             if (a.layout == MCT_QUAD) {
-                float4 q = Taps<DT>::quad(a.sino, ((size_t)j * (a.nv - 1) + vf) * ncol + uf);
+                float4 q = Taps<DT>::quad(a.sino, ((size_t)jl * (a.nv - 1) + vf) * ncol + uf);
                 acc += coef * bilerp(q, ru, rv);
             } else {
-                size_t r0 = ((size_t)vf * a.nviews + j) * a.nu + uf;
-                size_t r1 = r0 + (size_t)a.nviews * a.nu;
+                size_t r0 = ((size_t)vf * a.nvb + jl) * a.nu + uf;
+                size_t r1 = r0 + (size_t)a.nvb * a.nu;
                 float p0 = Taps<DT>::one(a.sino, r0) * (1.f - ru) + Taps<DT>::one(a.sino, r0 + 1) * ru;
                 float p1 = Taps<DT>::one(a.sino, r1) * (1.f - ru) + Taps<DT>::one(a.sino, r1 + 1) * ru;
                 acc += coef * ((1.f - rv) * p0 + rv * p1);
@@ -795,18 +827,18 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
         } else {
             float s0, s1;
             if (a.layout == MCT_PAIR) {
-                float2 p = Taps<DT>::pair(a.sino, ((size_t)iz * a.nviews + j) * ncol + uf);
+                float2 p = Taps<DT>::pair(a.sino, ((size_t)iz * a.nvb + jl) * ncol + uf);
                 s0 = p.x;
                 s1 = p.x + p.y;
             } else {
-                size_t r0 = ((size_t)iz * a.nviews + j) * a.nu + uf;
+                size_t r0 = ((size_t)iz * a.nvb + jl) * a.nu + uf;
                 s0 = Taps<DT>::one(a.sino, r0);
                 s1 = Taps<DT>::one(a.sino, r0 + 1);
             }
             acc += coef * ((1.f - ru) * s0 + ru * s1);
         }
     }
-    a.vol[id] = alive ? acc : 0.f;
+    a.vol[id] = bp_result(a, alive, acc);
 }
 
 // IR back-projection over all views (run_mgfbp_iterative_recon_IRN_CGLS.py:
@@ -1014,6 +1046,9 @@ BpArgs make_args(const mct_plan *p) {
     a.center_z = p->center_z;
     a.vt = p->bp_view;
     a.nxcd = p->ctx->xcd_count;
+    a.j0 = 0;
+    a.nvb = g.n_views;
+    a.accum = 0;
     return a;
 }
 
@@ -1110,8 +1145,15 @@ int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
     const size_t shmem = lds_bytes(cells, dtype);
     const int nblocks = a.tiles_x * a.tiles_y * a.nzc;
     mct_ctx *ctx = p->ctx;
-#define LDS_LAUNCH(DT, NQ) \
-    hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ>), dim3(nblocks), dim3(256), shmem, ctx->stream, a)
+#define LDS_LAUNCH(DT, NQ)                                                                          \
+    do {                                                                                            \
+        if (a.accum)                                                                                \
+            hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ, true>), dim3(nblocks), dim3(256), shmem, \
+                               ctx->stream, a);                                                     \
+        else                                                                                        \
+            hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ, false>), dim3(nblocks), dim3(256), shmem, \
+                               ctx->stream, a);                                                     \
+    } while (0)
 #define LDS_DT(NQ)                                         \
     if (dtype == MCT_F32) LDS_LAUNCH(MCT_F32, NQ);         \
     else if (dtype == MCT_F16) LDS_LAUNCH(MCT_F16, NQ);    \
@@ -1219,11 +1261,10 @@ int launch_batch(mct_ctx *ctx, const BpArgs *dargs, const int *doffs, int njobs,
     return MCT_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout, float *vol) {
+// FBP back-projection of views [j0, j0 + nvb) with volume mode accum (see
+// BpArgs); mct_backproject_fbp is the whole scan with accum 0.
+int backproject(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout, int j0, int nvb,
+                int accum, float *vol) {
     MCT_CHECK(plan && filt && vol, "NULL argument");
     const mct_geometry &g = plan->g;
     mct_ctx *ctx = plan->ctx;
@@ -1243,6 +1284,9 @@ int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t
     a.sino = filt;
     a.layout = layout;
     a.vol = vol;
+    a.j0 = j0;
+    a.nvb = nvb;
+    a.accum = accum;
     const bool fast = g.view_mode == 1 && !g.apply_pmatrix && layout != MCT_PLAIN;
     if (g.cone && layout == MCT_QUAD && ctx->bp_kernel != MCT_BP_DIRECT) {
         const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
@@ -1312,6 +1356,24 @@ int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t
     }
     MCT_LAUNCHED();
     return MCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mct_backproject_fbp(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout, float *vol) {
+    MCT_CHECK(plan, "NULL argument");
+    return backproject(plan, filt, dtype, layout, 0, plan->g.n_views, 0, vol);
+}
+
+int mct_backproject_views(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
+                          int32_t view_begin, int32_t view_count, int32_t finish, float *vol) {
+    MCT_CHECK(plan, "NULL argument");
+    MCT_CHECK(view_begin >= 0 && view_count >= 1 && view_count <= plan->g.n_views - view_begin,
+              "views [%d, %d + %d) outside the scan's %d views", view_begin, view_begin, view_count,
+              plan->g.n_views);
+    return backproject(plan, filt, dtype, layout, view_begin, view_count, finish ? 2 : 1, vol);
 }
 
 int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, float *vol) {

@@ -209,6 +209,21 @@ int mct_synchronize(mct_ctx *ctx) {
     return MCT_OK;
 }
 
+int mct_copy_views(mct_ctx *ctx, void *stream, const float *host_sgm, int32_t n_v, int32_t n_views,
+                   int32_t n_u, int32_t view_begin, int32_t view_count, float *dev_block) {
+    MCT_CHECK(ctx && host_sgm && dev_block, "NULL argument");
+    MCT_CHECK(n_v > 0 && n_u > 0, "empty sinogram");
+    MCT_CHECK(view_begin >= 0 && view_count >= 1 && view_count <= n_views - view_begin,
+              "views [%d, %d + %d) outside the sinogram's %d views", view_begin, view_begin,
+              view_count, n_views);
+    const size_t width = (size_t)view_count * n_u * sizeof(float);
+    const size_t pitch = (size_t)n_views * n_u * sizeof(float);
+    MCT_HIP(hipMemcpy2DAsync(dev_block, width, host_sgm + (size_t)view_begin * n_u, pitch, width,
+                             (size_t)n_v, hipMemcpyHostToDevice,
+                             stream ? (hipStream_t)stream : ctx->stream));
+    return MCT_OK;
+}
+
 int mct_set_bp_tile(mct_ctx *ctx, int32_t zc, int32_t batch) {
     if (!ctx) return mct::set_error(MCT_ERR_INVALID, "ctx is NULL");
     if (!(zc == 0 || zc == 4 || zc == 8 || zc == 16 || zc == 32))

@@ -15,10 +15,12 @@ struct BHCoef {
 // sinogram.  BHCorrection (run_mgfbp.py:946-954) then WeightSgm
 // (run_mgfbp.py:956-1011) operation by operation in FP32; the library is
 // built with -ffp-contract=off, so no product is fused into an FMA here.
+// The sinogram holds views [view0, view0 + nviews) of the scan (the Parker
+// weight reads each one's angle).
 __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int nu,
                               const float *__restrict__ uarr, const float *__restrict__ varr,
                               const float *__restrict__ ang, float sdd, float scan,
-                              int short_scan, int curved, BHCoef bh, int bh_order) {
+                              int short_scan, int curved, BHCoef bh, int bh_order, int view0) {
     // grid: x over elements j, y over views i, z over rows s
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y, s = blockIdx.z;
@@ -44,7 +46,7 @@ __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int n
         float ascan = fabsf(scan);
         float num_rounds = floorf(ascan / (kPI * 2.f));
         float remain = ascan - num_rounds * kPI * 2.f;
-        float beta = fabsf(ang[i] - ang[0]);
+        float beta = fabsf(ang[view0 + i] - ang[0]);
         float dir = ascan / scan;
         float gamma = curved ? ((-1.f) * u / sdd) * dir : atan2f((-1.f) * u, sdd) * dir;
         float gmax = remain - kPI;
@@ -160,21 +162,30 @@ __global__ void pack_plain_kernel(const float *__restrict__ in, int64_t n, void 
 
 extern "C" {
 
-int mct_weight_sinogram(mct_plan *plan, float *sgm, const float *bh_coef, int32_t bh_order) {
+int mct_weight_views(mct_plan *plan, float *sgm, int32_t view_begin, int32_t view_count,
+                     const float *bh_coef, int32_t bh_order) {
     MCT_CHECK(plan && sgm, "NULL argument");
     MCT_CHECK(bh_order >= 0 && bh_order <= kMaxBH, "bh_order must be in [0, %d]", kMaxBH);
     MCT_CHECK(bh_order == 0 || bh_coef, "bh_coef is NULL");
     const mct_geometry &g = plan->g;
+    MCT_CHECK(view_begin >= 0 && view_count >= 1 && view_count <= g.n_views - view_begin,
+              "views [%d, %d + %d) outside the scan's %d views", view_begin, view_begin, view_count,
+              g.n_views);
     mct_ctx *ctx = plan->ctx;
     BHCoef c = {};
     for (int t = 0; t < bh_order; ++t) c.c[t] = bh_coef[t];
-    MCT_CHECK(g.n_v <= 65535 && g.n_views <= 65535, "rows and views must be <= 65535");
-    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for(g.n_u, 256), g.n_views, g.n_v), dim3(256), 0,
+    MCT_CHECK(g.n_v <= 65535 && view_count <= 65535, "rows and views must be <= 65535");
+    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for(g.n_u, 256), view_count, g.n_v), dim3(256), 0,
                        ctx->stream,
-                       sgm, g.n_v, g.n_views, g.n_u, plan->u, plan->v, plan->angles, g.sdd,
-                       g.total_scan_angle, g.short_scan, g.curved, c, bh_order);
+                       sgm, g.n_v, view_count, g.n_u, plan->u, plan->v, plan->angles, g.sdd,
+                       g.total_scan_angle, g.short_scan, g.curved, c, bh_order, view_begin);
     MCT_LAUNCHED();
     return MCT_OK;
+}
+
+int mct_weight_sinogram(mct_plan *plan, float *sgm, const float *bh_coef, int32_t bh_order) {
+    MCT_CHECK(plan, "NULL argument");
+    return mct_weight_views(plan, sgm, 0, plan->g.n_views, bh_coef, bh_order);
 }
 
 int mct_filter_vertical(mct_ctx *ctx, const float *in, float *out, int32_t rows, int32_t views,

@@ -61,6 +61,23 @@ class Context:
     def synchronize(self) -> None:
         call("mct_synchronize", self.h)
 
+    def copy_views(self, host_sgm: torch.Tensor, view_begin: int, view_count: int,
+                   out: torch.Tensor, stream: torch.cuda.Stream | None = None) -> torch.Tensor:
+        """Views [view_begin, view_begin + view_count) of a host FP32 sinogram
+        (rows, views, n_u) into the device block out (rows, view_count, n_u),
+        asynchronously on `stream` (None: the context's stream).  host_sgm
+        must stay alive until the copy has run; pinned memory lets it overlap
+        kernels."""
+        if host_sgm.device.type != "cpu" or host_sgm.dtype != torch.float32 or not host_sgm.is_contiguous():
+            raise ValueError("host_sgm must be a contiguous float32 CPU tensor")
+        rows, views, nu = host_sgm.shape
+        if tuple(out.shape) != (rows, view_count, nu):
+            raise ValueError(f"out must have shape {(rows, view_count, nu)}, got {tuple(out.shape)}")
+        call("mct_copy_views", self.h, C.c_void_p(stream.cuda_stream) if stream is not None else None,
+             C.c_void_p(host_sgm.data_ptr()), rows, views, nu, int(view_begin), int(view_count),
+             _f32(out))
+        return out
+
     def set_bp_tile(self, zc: int, batch: int = 0) -> None:
         call("mct_set_bp_tile", self.h, int(zc), int(batch))
 
@@ -277,6 +294,26 @@ class Plan:
     def backproject_fbp(self, filt: torch.Tensor, dtype: str, layout: str,
                         vol: torch.Tensor) -> torch.Tensor:
         call("mct_backproject_fbp", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
+        return vol
+
+    def weight_views(self, sgm: torch.Tensor, view_begin: int, bh_coef=None) -> torch.Tensor:
+        """weight() of a block (rows, view_count, n_u) holding views
+        [view_begin, view_begin + view_count) of the scan."""
+        n = int(sgm.shape[1])
+        if bh_coef is None or len(bh_coef) == 0:
+            call("mct_weight_views", self.h, _f32(sgm), int(view_begin), n, None, 0)
+        else:
+            keep, cp = _host_f32(bh_coef)
+            call("mct_weight_views", self.h, _f32(sgm), int(view_begin), n, cp, len(keep))
+        return sgm
+
+    def backproject_views(self, filt: torch.Tensor, dtype: str, layout: str, view_begin: int,
+                          view_count: int, vol: torch.Tensor, finish: bool = False) -> torch.Tensor:
+        """Adds the back-projection of views [view_begin, view_begin +
+        view_count) (filt: their packed block) to vol; finish on the last
+        block clears the truncation marks."""
+        call("mct_backproject_views", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout],
+             int(view_begin), int(view_count), int(bool(finish)), _f32(vol))
         return vol
 
     def forward_project(self, vol: torch.Tensor, out: torch.Tensor, ray_mask=None,

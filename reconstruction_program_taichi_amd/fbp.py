@@ -490,30 +490,43 @@ class Mgfbp:
             self._plan = Plan(self.ctx(), self.geometry_spec())
         return self._plan
 
+    def tap_layout(self) -> str:
+        """Packed tap layout of the back-projector: QUAD (2 x 2 cells) for cone
+        beam, PAIR (neighbouring elements of a row) for fan beam."""
+        return "quad" if self.cone_beam else "pair"
+
+    def weight_and_filter(self, sgm: torch.Tensor, view_begin: int | None = None) -> torch.Tensor:
+        """BHCorrection + WeightSgm + FilterSinogram (run_mgfbp.py:63-70,
+        1282-1292) on a device FP32 sinogram (rows, views, n_u), weighted in
+        place; with view_begin, sgm is the block of views [view_begin,
+        view_begin + views) of the scan.  Returns the filtered sinogram."""
+        ctx, plan = self.ctx(), self.plan()
+        bh = self.array_bh_coefficients if self.bool_bh_correction else None
+        if view_begin is None:
+            plan.weight(sgm, bh)
+        else:
+            plan.weight_views(sgm, view_begin, bh)
+        taps = self.recon_kernel()
+        if taps is None:
+            # run_mgfbp.py:1283-1285: kernel 'None' filters nothing, the
+            # vertical Gaussian included
+            return sgm
+        src = sgm
+        if self.apply_gauss_vertical:
+            src = torch.empty_like(sgm)
+            ctx.filter_vertical(sgm, src, self.vertical_kernel(), self.det_elem_height)
+        filt = torch.empty_like(src)
+        ctx.filter_rows(src, filt, taps, self.det_elem_width, self.filter_method)
+        return filt
+
     def reconstruct(self, sgm: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """The device pipeline of MainFunction (run_mgfbp.py:63-81) on a
         resident FP32 sinogram (rows, views, n_u); sgm is weighted in place
         (as the reference mutates its ndarray argument)."""
         ctx, plan = self.ctx(), self.plan()
-        if self.bool_bh_correction:
-            plan.weight(sgm, self.array_bh_coefficients)
-        else:
-            plan.weight(sgm)
-        src = sgm
-        if self.apply_gauss_vertical:
-            tmp = torch.empty_like(sgm)
-            ctx.filter_vertical(sgm, tmp, self.vertical_kernel(), self.det_elem_height)
-            src = tmp
-        taps = self.recon_kernel()
-        if taps is None:
-            filt = src
-        else:
-            filt = torch.empty_like(src)
-            ctx.filter_rows(src, filt, taps, self.det_elem_width, self.filter_method)
+        filt = self.weight_and_filter(sgm)
         self.last_filtered = filt
-        layout = "quad" if self.cone_beam else "pair"
-        if self.recon_view_mode != 1 and not self.cone_beam:
-            layout = "pair"
+        layout = self.tap_layout()
         packed = ctx.pack(filt, self.storage, layout)
         if out is None:
             out = torch.empty((self.img_dim_z, self.img_dim, self.img_dim), dtype=torch.float32,
@@ -542,19 +555,28 @@ class Mgfbp:
         if self.output_file_format in ("tif", "tiff"):
             self.output_file = re.sub(".raw", ".tif", self.output_file)
         self.output_path = os.path.join(self.output_dir, self.output_file_prefix + self.output_file)
+        self.load_rows()
+        return True
+
+    def load_rows(self) -> None:
+        """The first view_num views of the file's recon-range rows into
+        self.img_sgm (rows, view_num, n_u)."""
+        self.img_sgm = np.ascontiguousarray(self.read_file_rows()[:, 0:self.view_num, :])
+
+    def read_file_rows(self) -> np.ndarray:
+        """The detector rows of the recon range of self.input_path as
+        (rows, sgm_height, n_u) FP32 (run_mgfbp.py:1206-1222): a sinogram file
+        is [rows][views][n_u], a post-log image file [views][rows][n_u]."""
         nu, rows = self.det_elem_count_horizontal, self.det_elem_count_vertical_actual
         if self.input_file_form == "sinogram":
             off = self.det_elem_vertical_recon_range_begin * 4 * self.sgm_height * nu
-            buf = np.fromfile(self.input_path, dtype=np.float32, offset=off,
-                              count=rows * self.sgm_height * nu).reshape(rows, self.sgm_height, nu)
-        else:
-            off = self.det_elem_vertical_recon_range_begin * nu * 4
-            gap = (self.det_elem_vertical_recon_range_begin + self.det_elem_count_vertical - 1
-                   - self.det_elem_vertical_recon_range_end) * nu * 4
-            buf = imreadRaw(self.input_path, height=rows, width=nu, offset=off, gap=gap,
-                            nSlice=self.sgm_height).transpose(1, 0, 2)
-        self.img_sgm = np.ascontiguousarray(buf[:, 0:self.view_num, :])
-        return True
+            return np.fromfile(self.input_path, dtype=np.float32, offset=off,
+                               count=rows * self.sgm_height * nu).reshape(rows, self.sgm_height, nu)
+        off = self.det_elem_vertical_recon_range_begin * nu * 4
+        gap = (self.det_elem_vertical_recon_range_begin + self.det_elem_count_vertical - 1
+               - self.det_elem_vertical_recon_range_end) * nu * 4
+        return imreadRaw(self.input_path, height=rows, width=nu, offset=off, gap=gap,
+                         nSlice=self.sgm_height).transpose(1, 0, 2)
 
     def SaveFilteredSinogram(self):
         if self.save_filtered_sinogram:

@@ -148,16 +148,16 @@ class Mgfbp_helical(Mgfbp):
             bh = self.array_bh_coefficients if self.bool_bh_correction else None
             for z, plan in enumerate(plans):
                 plan.weight(reb[z:z + 1], bh)
+            taps = self.recon_kernel()
             src = reb
-            if self.apply_gauss_vertical:  # one detector row per slice
+            if self.apply_gauss_vertical and taps is not None:  # one detector row per slice
                 tmp = torch.empty_like(reb)
                 ctx.filter_vertical(reb.view(1, nz * vc, n), tmp.view(1, nz * vc, n),
                                     geo.gaussian_taps(1, self.det_elem_height or 1.0,
                                                       self.det_elem_vertical_gauss_filter_size),
                                     self.det_elem_height)
                 src = tmp
-            taps = self.recon_kernel()
-            if taps is None:
+            if taps is None:  # run_mgfbp.py:1283-1285: kernel 'None' filters nothing
                 filt = src
             else:
                 filt = torch.empty_like(src)

@@ -113,8 +113,11 @@ def test_full_scan_weight_is_cosine_only():
 
 
 # -------------------------------------------------------- back-projection
-def _bp_brute(filt, g):
-    """Scalar loops of run_mgfbp.py:1086-1182 on a tiny grid (FP32 scalars)."""
+def _bp_brute(filt, g, v2=False, vol=None):
+    """Scalar loops of run_mgfbp.py:1086-1182 on a tiny grid (FP32 scalars).
+    v2: the view-by-view form of run_mgfbp_v2.py:71-92, 232-364 instead (start
+    from vol, a failed range test sets -10000 and the loop goes on, voxels
+    below -5000 end as 0)."""
     f32 = np.float32
     nx, nz = g["img_dim"], g["img_dim_z"]
     nrows, nviews, nu = filt.shape
@@ -128,12 +131,15 @@ def _bp_brute(filt, g):
                 x = f32(g["pix"]) * (f32(ix) - f32((nx - 1) / 2.0))
                 y = (-f32(g["pix"])) * (f32(iy) - f32((nx - 1) / 2.0))
                 z = (f32(iz) - f32((nz - 1) / 2.0)) * f32(g["vox_h"]) + f32(g.get("cz", 0.0))
-                acc = f32(0)
+                acc = f32(0) if vol is None else f32(vol[iz, iy, ix])
                 for j in range(nviews):
                     L = f32(g["sid"]) - x * c[j] - y * s[j]
                     mag = f32(g["sdd"]) / L
                     ui = (mag * ((-x) * s[j] + y * c[j]) - u[0]) / (u[1] - u[0])
                     if ui < 0 or ui + f32(1) > f32(nu - 1):
+                        if v2:
+                            acc = f32(-10000)
+                            continue
                         acc = None
                         break
                     uf = int(np.floor(ui))
@@ -141,12 +147,17 @@ def _bp_brute(filt, g):
                     vi = (mag * z - v[0]) / f32(g["det_h"]) * np.abs(v[1] - v[0]) / (v[1] - v[0])
                     vf = int(np.floor(vi))
                     if vf < 0 or vf + 1 > nrows - 1:
+                        if v2:
+                            acc = f32(-10000)
+                            continue
                         acc = None
                         break
                     rv = vi - f32(vf)
                     p0 = filt[vf, j, uf] * (f32(1) - ru) + filt[vf, j, uf + 1] * ru
                     p1 = filt[vf + 1, j, uf] * (f32(1) - ru) + filt[vf + 1, j, uf + 1] * ru
                     acc = acc + (f32(g["sid"]) * (f32(1) / (L * L))) * ((f32(1) - rv) * p0 + rv * p1) * d[j] * div
+                if v2 and acc < f32(-5000):
+                    acc = f32(0)
                 out[iz, iy, ix] = 0 if acc is None else acc
     return out
 
@@ -159,6 +170,36 @@ def test_bp_oracle_matches_brute_force_tiny():
     a = recon.backproject_fbp(f, g, np.float32)
     b = _bp_brute(f, g)
     assert np.array_equal(a, b)
+
+
+def test_bp_v2_oracle_matches_brute_force_tiny():
+    """The view-by-view FBP: equal to the one-pass FBP on ordinary data; with
+    values large enough for the -10000 mark and the -5000 clearing to show,
+    equal to the scalar loop of run_mgfbp_v2.py (a voxel masked in an early
+    view only ends as -10000 plus its later views; sums below -5000 end as 0;
+    the views add to the volume they are given)."""
+    spec = cone_case(nx=6, nz=5, nv=8, nu=20, nrows=10, pix=2.0, vh=2.0, w=1.6, h=1.6, sid=60.0,
+                     sdd=100.0)
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((spec.n_v, spec.n_views, spec.n_u)).astype(np.float32)
+    one, alive = recon.backproject_fbp(f, g, np.float32, return_mask=True)
+    assert 0 < alive.sum() < alive.size  # some voxels leave the detector
+    assert np.array_equal(recon.backproject_fbp_v2(f, g), one)
+    assert np.array_equal(recon.backproject_fbp_v2(f, g), _bp_brute(f, g, v2=True))
+    big = (f + 4.0) * np.float32(4e5)  # positive: masked voxels climb back above -5000
+    a = recon.backproject_fbp_v2(big, g)
+    assert np.array_equal(a, _bp_brute(big, g, v2=True))
+    assert np.any((a != 0) & ~alive)
+    neg = -big  # negative: every sum drops below -5000
+    b = recon.backproject_fbp_v2(neg, g)
+    assert np.array_equal(b, _bp_brute(neg, g, v2=True))
+    full = recon.backproject_fbp(neg, g, np.float32)
+    assert np.array_equal(b, np.where(full < -5000, np.float32(0), full)) and not b.any()
+    vol = rng.uniform(-1, 1, one.shape).astype(np.float32)
+    c = recon.backproject_fbp_v2(f, g, vol=vol)
+    assert np.array_equal(c, _bp_brute(f, g, v2=True, vol=vol))
+    assert not np.array_equal(c, one)
 
 
 @pytest.mark.parametrize("kind", ["cone", "fan"])
