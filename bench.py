@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
 # (FETCH x2, WRITE x1); profiles/fdk_c1_kernels.md.  Keyed by
 # (kernel, storage, --bp-tile, --bp-batch); other settings report null.
-MEASURED_TRAFFIC = {("direct", "f32", 0, 0): 94.2e9}
+MEASURED_TRAFFIC = {("direct", "f32", 0, 0): 94.2e9, ("lds", "f32", 0, 0): 77.6e9}
 # Per-CU fill rates on CDNA: vector L1 64 B/clk, LDS 128 B/clk; MI355X peak
 # engine clock 2.4 GHz.
 GATHER_BYTES_PER_CLK_PER_CU = {"direct": 64, "lds": 128}
