@@ -161,8 +161,11 @@ int mct_filter_vertical(mct_ctx *ctx, const float *in, float *out, int32_t rows,
  * of length n: the full, non-circular linear convolution, i.e. a Toeplitz
  * product.  in/out DEVICE FP32 [n_rows][n]; taps HOST array of 2n-1 floats
  * (GenerateHammingKernel run_mgfbp.py:877-914 or its Gaussian-apodised form).
- * method: mct_filter_method.  For the MFMA methods the Toeplitz panel is
- * cached in the context and rebuilt when taps, n or method change. */
+ * method: mct_filter_method.  For the MFMA methods the split taps are
+ * cached in the context and rebuilt when taps or n change, and the input rows
+ * are first converted into BF16 planes held in context scratch (2 bytes per
+ * padded sample, 4 for the split method; it grows to the largest call), so
+ * calls on one context must be ordered, as calls on its stream are. */
 int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, int32_t n,
                     const float *taps, float scale, int32_t method);
 

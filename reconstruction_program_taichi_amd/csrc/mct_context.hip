@@ -182,6 +182,7 @@ int mct_destroy(mct_ctx *ctx) {
     hipStreamSynchronize(ctx->stream);
     if (ctx->partials) hipFree(ctx->partials);
     if (ctx->toep) hipFree(ctx->toep);
+    if (ctx->fsplit) hipFree(ctx->fsplit);
     if (ctx->taps_dev) hipFree(ctx->taps_dev);
     if (ctx->vkern_dev) hipFree(ctx->vkern_dev);
     if (ctx->aux_dev) hipFree(ctx->aux_dev);

@@ -2,18 +2,23 @@
 // run_mgfbp.py:1033-1046):  OUT[r, k] = scale * sum_m IN[r, m] T[m, k],
 // T[m, k] = taps[k + n-1 - m].
 //
-// Workgroup tile: 128 rows x 128 output columns, four wavefronts of 64 x 64,
-// each a 4 x 4 grid of v_mfma_f32_16x16x32_bf16 accumulators (FP32).  Per K
-// step of 32:
-//   * A (128 rows x 32 samples, FP32) is read from HBM, rounded to BF16 (hi)
-//     and, for the split method, the remainder x - hi rounded again (lo), and
-//     stored in LDS ([row][k], rows padded to 80 bytes);
-//   * B is never materialised: every B fragment of the tile is 8 consecutive
-//     entries of the reversed tap window R[i] = taps[top - i] (159 taps for a
-//     128 x 32 tile), kept in LDS as 8 copies shifted by 0..7 so each
-//     fragment is one aligned 16-byte LDS read.
+// Two launches:
+//   * split_bf16_kernel converts every input row once into BF16 planes
+//     [rows][npad]: hi = bf16(x) and, for the split method, lo = bf16(x - hi),
+//     zero past n;
+//   * ramp_gemm_kernel: workgroup tile 128 rows x 128 output columns, four
+//     wavefronts of 64 x 64, each a 4 x 4 grid of v_mfma_f32_16x16x32_bf16
+//     accumulators (FP32).  Per K step of 32 the A tile (128 rows x 32
+//     samples of each plane) is copied from the planes into one of two LDS
+//     buffers; the copy for step s+1 is in flight while step s's MFMAs run.
+//     B is never materialised: every B fragment is 8 consecutive entries of
+//     the reversed tap window W[i] = taps[T - i], T = k0 + 127 + n-1, kept in
+//     LDS as 8 copies shifted by 0..7 so each fragment is one aligned 16-byte
+//     LDS read; a window covers KC samples of K and is restaged every KC/32
+//     steps.  Workgroup ids are remapped so the 8 column tiles of a row tile
+//     run on one XCD and share its L2.
 // BF16: hi x hi.  BF16X3: hi x hi + hi x lo + lo x hi (relative product error
-// about 2^-16 instead of 2^-8).
+// about 2^-16 instead of 2^-8), in that order for every accumulator.
 //
 // MFMA operand layout (16x16x32, 64 lanes): lane l holds A[l % 16][8(l/16) .. +7],
 // B[8(l/16) .. +7][l % 16]; the 4 accumulators of lane l are
@@ -30,29 +35,64 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TM = 128, TN = 128, TK = 32;
-constexpr int APAD = 40;   // LDS row stride of the A tile, in BF16 elements (80 B)
-constexpr int WLEN = 160;  // entries of one shifted window copy (needs 159 + alignment)
+constexpr int APAD = 40;      // LDS row stride of an A tile, in BF16 elements (80 B)
+constexpr int KC = 256;       // K samples covered by one staged tap window
+constexpr int WLC = KC + TN;  // entries per shifted copy: fragments start below KC - TK + TN + 24
 
 __device__ __forceinline__ __bf16 to_bf16(float f) {
     return __builtin_bit_cast(__bf16, mct::f32_to_bf16_rn(f));
 }
 
+// One thread per 8 samples of a row: in[r][m0 .. m0+7] -> hi/lo planes.
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void ramp_mfma_kernel(const float *__restrict__ in,
+__global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict__ in, int64_t rows,
+                                                         int n, int npad, __bf16 *__restrict__ hi,
+                                                         __bf16 *__restrict__ lo) {
+    const int groups = npad / 8;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= rows * groups) return;
+    const int64_t r = t / groups;
+    const int m0 = (int)(t - r * groups) * 8;
+    const float *src = in + r * n + m0;
+    float x[8];
+    if (m0 + 8 <= n && ((((uintptr_t)src) & 15) == 0)) {
+        const float4 a = reinterpret_cast<const float4 *>(src)[0];
+        const float4 b = reinterpret_cast<const float4 *>(src)[1];
+        x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w;
+        x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = m0 + q < n ? src[q] : 0.f;
+    }
+    bf16x8 h, l;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        h[q] = to_bf16(x[q]);
+        if (SPLIT) l[q] = to_bf16(x[q] - (float)h[q]);
+    }
+    *reinterpret_cast<bf16x8 *>(hi + r * npad + m0) = h;
+    if (SPLIT) *reinterpret_cast<bf16x8 *>(lo + r * npad + m0) = l;
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void ramp_gemm_kernel(const __bf16 *__restrict__ plane_hi,
+                                                        const __bf16 *__restrict__ plane_lo,
                                                         float *__restrict__ out, int64_t rows, int n,
-                                                        int npad, const __bf16 *__restrict__ taps_hi,
-                                                        const __bf16 *__restrict__ taps_lo,
-                                                        float scale) {
-    __shared__ __attribute__((aligned(16))) __bf16 a_hi[TM * APAD];
-    __shared__ __attribute__((aligned(16))) __bf16 a_lo[SPLIT ? TM * APAD : 8];
-    __shared__ __attribute__((aligned(16))) __bf16 w_hi[8 * WLEN];
-    __shared__ __attribute__((aligned(16))) __bf16 w_lo[SPLIT ? 8 * WLEN : 8];
+                                                        int npad, int col_tiles, int nxcd,
+                                                        const __bf16 *__restrict__ taps_hi,
+                                                        const __bf16 *__restrict__ taps_lo, float scale) {
+    __shared__ __attribute__((aligned(16))) __bf16 a_hi[2][TM * APAD];
+    __shared__ __attribute__((aligned(16))) __bf16 a_lo[2][SPLIT ? TM * APAD : 8];
+    __shared__ __attribute__((aligned(16))) __bf16 w_hi[8 * WLC];
+    __shared__ __attribute__((aligned(16))) __bf16 w_lo[SPLIT ? 8 * WLC : 8];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const int64_t r0 = (int64_t)blockIdx.y * TM;
-    const int k0 = blockIdx.x * TN;
+    const int tile = mct::xcd_remap((int)blockIdx.x, (int)gridDim.x, nxcd);
+    const int64_t r0 = (int64_t)(tile / col_tiles) * TM;
+    const int k0 = (tile % col_tiles) * TN;
     const int ntaps = 2 * n - 1;
+    const int T = k0 + (TN - 1) + n - 1;
 
     f32x4 acc[4][4];
 #pragma unroll
@@ -60,71 +100,71 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const float *__restrict_
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // A staging: thread t covers row t/2, samples 16*(t%2) .. +15 of the K step
+    // A copy: thread t moves row t/2, samples 16 (t % 2) .. +15 of a K step
+    // (two 16-byte pieces of each plane)
     const int arow = tid >> 1, acol = (tid & 1) * 16;
     const int64_t grow = r0 + arow;
-
-    for (int m0 = 0; m0 < npad; m0 += TK) {
-        {
-            float x[16];
-            const bool rok = grow < rows;
-            const float *src = in + grow * n + m0 + acol;
-            if (rok && m0 + acol + 16 <= n && ((((uintptr_t)src) & 15) == 0)) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float4 v = reinterpret_cast<const float4 *>(src)[q];
-                    x[4 * q] = v.x;
-                    x[4 * q + 1] = v.y;
-                    x[4 * q + 2] = v.z;
-                    x[4 * q + 3] = v.w;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) x[q] = (rok && m0 + acol + q < n) ? src[q] : 0.f;
-            }
-            bf16x8 h0, h1, l0, l1;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                __bf16 a = to_bf16(x[q]), b = to_bf16(x[q + 8]);
-                h0[q] = a;
-                h1[q] = b;
-                if (SPLIT) {
-                    l0[q] = to_bf16(x[q] - (float)a);
-                    l1[q] = to_bf16(x[q + 8] - (float)b);
-                }
-            }
-            *reinterpret_cast<bf16x8 *>(&a_hi[arow * APAD + acol]) = h0;
-            *reinterpret_cast<bf16x8 *>(&a_hi[arow * APAD + acol + 8]) = h1;
+    const bool rok = grow < rows;
+    const __bf16 *ph = plane_hi + (rok ? grow : 0) * npad + acol;
+    const __bf16 *pl = plane_lo + (rok ? grow : 0) * npad + acol;
+    const bf16x8 zero8 = {};
+    bf16x8 sh0 = zero8, sh1 = zero8, sl0 = zero8, sl1 = zero8;
+    auto load_a = [&](int m0) {
+        if (rok) {
+            sh0 = *reinterpret_cast<const bf16x8 *>(ph + m0);
+            sh1 = *reinterpret_cast<const bf16x8 *>(ph + m0 + 8);
             if (SPLIT) {
-                *reinterpret_cast<bf16x8 *>(&a_lo[arow * APAD + acol]) = l0;
-                *reinterpret_cast<bf16x8 *>(&a_lo[arow * APAD + acol + 8]) = l1;
+                sl0 = *reinterpret_cast<const bf16x8 *>(pl + m0);
+                sl1 = *reinterpret_cast<const bf16x8 *>(pl + m0 + 8);
             }
         }
-        // reversed tap window, 8 shifted copies: w[c][t] = taps[top - (t + c)]
-        const int top = k0 - m0 + (TN - 1) + n - 1;
-        for (int e = tid; e < 8 * WLEN; e += 256) {
-            const int c = e / WLEN, t = e - c * WLEN;
-            const int idx = top - (t + c);
+    };
+    auto store_a = [&](int b) {
+        *reinterpret_cast<bf16x8 *>(&a_hi[b][arow * APAD + acol]) = sh0;
+        *reinterpret_cast<bf16x8 *>(&a_hi[b][arow * APAD + acol + 8]) = sh1;
+        if (SPLIT) {
+            *reinterpret_cast<bf16x8 *>(&a_lo[b][arow * APAD + acol]) = sl0;
+            *reinterpret_cast<bf16x8 *>(&a_lo[b][arow * APAD + acol + 8]) = sl1;
+        }
+    };
+    // w[c][t] = W[mb + t + c]
+    auto load_window = [&](int mb) {
+        for (int e = tid; e < 8 * WLC; e += 256) {
+            const int c = e / WLC, t = e - c * WLC;
+            const int idx = T - mb - t - c;
             const bool ok = idx >= 0 && idx < ntaps;
             w_hi[e] = ok ? taps_hi[idx] : __builtin_bit_cast(__bf16, (uint16_t)0);
             if (SPLIT) w_lo[e] = ok ? taps_lo[idx] : __builtin_bit_cast(__bf16, (uint16_t)0);
         }
-        __syncthreads();
+    };
+
+    load_a(0);
+    store_a(0);
+    const int nsteps = npad / TK;
+    const int kb = (lane >> 4) * 8;
+    for (int step = 0; step < nsteps; ++step) {
+        const int m0 = step * TK, cur = step & 1;
+        const int mw = m0 % KC;  // position in the staged window
+        if (mw == 0) {
+            if (step) __syncthreads();  // every wave is done with the previous window
+            load_window(m0);
+        }
+        __syncthreads();  // A buffer `cur` and the window are in LDS
+        if (step + 1 < nsteps) load_a(m0 + TK);
         bf16x8 ah[4], al[4], bh[4], bl[4];
-        const int kb = (lane >> 4) * 8;
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb) {
             const int row = wr * 64 + rb * 16 + (lane & 15);
-            ah[rb] = *reinterpret_cast<const bf16x8 *>(&a_hi[row * APAD + kb]);
-            if (SPLIT) al[rb] = *reinterpret_cast<const bf16x8 *>(&a_lo[row * APAD + kb]);
+            ah[rb] = *reinterpret_cast<const bf16x8 *>(&a_hi[cur][row * APAD + kb]);
+            if (SPLIT) al[rb] = *reinterpret_cast<const bf16x8 *>(&a_lo[cur][row * APAD + kb]);
         }
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
             const int jj = wc * 64 + cb * 16 + (lane & 15);
-            const int s = (TN - 1) - jj + kb;  // B[kb + e][jj] = R[s + e]
-            const int c = s & 7;
-            bh[cb] = *reinterpret_cast<const bf16x8 *>(&w_hi[c * WLEN + (s - c)]);
-            if (SPLIT) bl[cb] = *reinterpret_cast<const bf16x8 *>(&w_lo[c * WLEN + (s - c)]);
+            const int p = mw + (TN - 1) - jj + kb;  // B[kb + e][jj] = W[m0 + p' + e], p' = p - mw
+            const int c = p & 7;
+            bh[cb] = *reinterpret_cast<const bf16x8 *>(&w_hi[c * WLC + (p - c)]);
+            if (SPLIT) bl[cb] = *reinterpret_cast<const bf16x8 *>(&w_lo[c * WLC + (p - c)]);
         }
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
@@ -136,7 +176,7 @@ __global__ __launch_bounds__(256) void ramp_mfma_kernel(const float *__restrict_
                     acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh[cb], acc[rb][cb], 0, 0, 0);
                 }
             }
-        __syncthreads();
+        if (step + 1 < nsteps) store_a(cur ^ 1);  // no wave reads buffer cur^1 after this step's barrier
     }
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
@@ -204,16 +244,41 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
     }
     const __bf16 *thi = reinterpret_cast<const __bf16 *>(ctx->toep);
     const __bf16 *tlo = thi + ntaps;
+    const bool split = method == MCT_FILTER_BF16X3_MFMA;
     const int npad = (n + TK - 1) / TK * TK;
     const int64_t row_tiles = (n_rows + TM - 1) / TM;
-    MCT_CHECK(row_tiles <= 65535, "too many rows for one launch (%lld)", (long long)n_rows);
-    dim3 grid((n + TN - 1) / TN, (unsigned)row_tiles);
-    if (method == MCT_FILTER_BF16X3_MFMA)
-        hipLaunchKernelGGL(ramp_mfma_kernel<true>, grid, dim3(256), 0, ctx->stream, in, out, n_rows, n,
-                           npad, thi, tlo, scale);
-    else
-        hipLaunchKernelGGL(ramp_mfma_kernel<false>, grid, dim3(256), 0, ctx->stream, in, out, n_rows,
-                           n, npad, thi, tlo, scale);
+    const int col_tiles = (n + TN - 1) / TN;
+    MCT_CHECK(row_tiles * col_tiles <= INT32_MAX, "too many rows for one launch (%lld)",
+              (long long)n_rows);
+    // BF16 planes of the input rows (context scratch, kept between calls)
+    const int64_t plane = n_rows * (int64_t)npad;
+    const int64_t bytes = plane * 2 * (split ? 2 : 1);
+    if (ctx->fsplit_bytes < bytes) {
+        if (ctx->fsplit) {
+            MCT_HIP(hipStreamSynchronize(ctx->stream));
+            MCT_HIP(hipFree(ctx->fsplit));
+            ctx->fsplit = nullptr;
+            ctx->fsplit_bytes = 0;
+        }
+        MCT_HIP(hipMalloc(&ctx->fsplit, bytes));
+        ctx->fsplit_bytes = bytes;
+    }
+    __bf16 *phi = reinterpret_cast<__bf16 *>(ctx->fsplit);
+    __bf16 *plo = split ? phi + plane : phi;
+    const int64_t groups = n_rows * (npad / 8);
+    const dim3 sgrid((unsigned)((groups + 255) / 256));
+    const dim3 grid((unsigned)(row_tiles * col_tiles));
+    if (split) {
+        hipLaunchKernelGGL(split_bf16_kernel<true>, sgrid, dim3(256), 0, ctx->stream, in, n_rows, n,
+                           npad, phi, plo);
+        hipLaunchKernelGGL(ramp_gemm_kernel<true>, grid, dim3(256), 0, ctx->stream, phi, plo, out,
+                           n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, scale);
+    } else {
+        hipLaunchKernelGGL(split_bf16_kernel<false>, sgrid, dim3(256), 0, ctx->stream, in, n_rows, n,
+                           npad, phi, plo);
+        hipLaunchKernelGGL(ramp_gemm_kernel<false>, grid, dim3(256), 0, ctx->stream, phi, plo, out,
+                           n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, scale);
+    }
     MCT_LAUNCHED();
     return MCT_OK;
 }

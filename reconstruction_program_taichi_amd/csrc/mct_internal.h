@@ -45,6 +45,9 @@ struct mct_ctx {
     int toep_n = 0, toep_method = -1;
     float toep_scale = 0.f;
     uint64_t toep_hash = 0;
+    // split-BF16 planes (hi, lo) of the MFMA filter's input rows
+    void *fsplit = nullptr;
+    int64_t fsplit_bytes = 0;
     float *taps_dev = nullptr;  // device copy of the ramp taps (direct method)
     int64_t taps_cap = 0;
     uint64_t taps_hash = 0;

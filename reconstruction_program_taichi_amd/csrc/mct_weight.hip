@@ -11,22 +11,14 @@ struct BHCoef {
     float c[kMaxBH];
 };
 
-// One thread per (row s, view i, element j) of a PLAIN [n_v][n_views][n_u]
-// sinogram.  BHCorrection (run_mgfbp.py:946-954) then WeightSgm
-// (run_mgfbp.py:956-1011) operation by operation in FP32; the library is
-// built with -ffp-contract=off, so no product is fused into an FMA here.
-// The sinogram holds views [view0, view0 + nviews) of the scan (the Parker
-// weight reads each one's angle).
-__global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int nu,
-                              const float *__restrict__ uarr, const float *__restrict__ varr,
-                              const float *__restrict__ ang, float sdd, float scan,
-                              int short_scan, int curved, BHCoef bh, int bh_order, int view0) {
-    // grid: x over elements j, y over views i, z over rows s
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y, s = blockIdx.z;
-    if (j >= nu) return;
-    const size_t idx = ((size_t)s * nviews + i) * nu + j;
-    float p = sgm[idx];
+// BHCorrection (run_mgfbp.py:946-954) then WeightSgm (run_mgfbp.py:956-1011)
+// of one sample p at detector element j of row s in scan view i, operation
+// by operation in FP32; the library is built with -ffp-contract=off, so no
+// product is fused into an FMA here.
+__device__ __forceinline__ float weigh(float p, int j, int s, int i, const float *__restrict__ uarr,
+                                       const float *__restrict__ varr, const float *__restrict__ ang,
+                                       float sdd, float scan, int short_scan, int curved,
+                                       const BHCoef &bh, int bh_order) {
     if (bh_order > 0) {
         float acc = 0.f, pw = p;
         for (int t = 0; t < bh_order; ++t) {
@@ -46,7 +38,7 @@ __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int n
         float ascan = fabsf(scan);
         float num_rounds = floorf(ascan / (kPI * 2.f));
         float remain = ascan - num_rounds * kPI * 2.f;
-        float beta = fabsf(ang[view0 + i] - ang[0]);
+        float beta = fabsf(ang[i] - ang[0]);
         float dir = ascan / scan;
         float gamma = curved ? ((-1.f) * u / sdd) * dir : atan2f((-1.f) * u, sdd) * dir;
         float gmax = remain - kPI;
@@ -76,7 +68,41 @@ __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int n
         }
         p *= w;
     }
-    sgm[idx] = p;
+    return p;
+}
+
+// Four consecutive elements per thread of a PLAIN [n_v][n_views][n_u]
+// sinogram holding views [view0, view0 + nviews) of the scan (the Parker
+// weight reads each one's angle).
+__global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int nu,
+                              const float *__restrict__ uarr, const float *__restrict__ varr,
+                              const float *__restrict__ ang, float sdd, float scan,
+                              int short_scan, int curved, BHCoef bh, int bh_order, int view0) {
+    // grid: x over groups of 4 elements, y over views i, z over rows s
+    const int j0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const int i = blockIdx.y, s = blockIdx.z;
+    if (j0 >= nu) return;
+    float *row = sgm + ((size_t)s * nviews + i) * nu;
+    const bool vec = j0 + 4 <= nu && ((((uintptr_t)(row + j0)) & 15) == 0);
+    float p[4];
+    if (vec) {
+        const float4 v = *reinterpret_cast<const float4 *>(row + j0);
+        p[0] = v.x, p[1] = v.y, p[2] = v.z, p[3] = v.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p[q] = j0 + q < nu ? row[j0 + q] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        p[q] = weigh(p[q], j0 + q < nu ? j0 + q : nu - 1, s, view0 + i, uarr, varr, ang, sdd, scan,
+                     short_scan, curved, bh, bh_order);
+    if (vec) {
+        *reinterpret_cast<float4 *>(row + j0) = make_float4(p[0], p[1], p[2], p[3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (j0 + q < nu) row[j0 + q] = p[q];
+    }
 }
 
 // run_mgfbp.py:1019-1031 (vertical Gaussian, +-10 rows, ascending n).
@@ -100,30 +126,40 @@ template <int DT> __device__ __forceinline__ uint16_t to_half_bits(float f) {
 
 // QUAD: out[j][r][c] = bilinear coefficients of cell (r, c) of view j:
 // {a, b - a, c - a, d - c - b + a} with a = in[r][j][c], b = in[r][j][c+1],
-// c = in[r+1][j][c], d = in[r+1][j][c+1] (each rounded once from FP64)
+// c = in[r+1][j][c], d = in[r+1][j][c+1] (each rounded once from FP64).
+// A thread walks kQuadRows cell rows of one column, so every detector row it
+// reads serves two cells.
+constexpr int kQuadRows = 16;
 template <int DT>
 __global__ void pack_quad_kernel(const float *__restrict__ in, int rows, int views, int nu,
                                  void *__restrict__ out) {
-    // grid: x over detector cells c, y over rows r, z over views j
+    // grid: x over detector cells c, y over views j, z over chunks of cell rows
     const int nc = nu - 1, nr = rows - 1;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y, j = blockIdx.z;
+    const int j = blockIdx.y, rb = blockIdx.z * kQuadRows;
     if (c >= nc) return;
-    const float *p0 = in + ((int64_t)r * views + j) * nu + c;
-    const float *p1 = p0 + (int64_t)views * nu;
-    const double a = p0[0], b = p0[1], cc = p1[0], d = p1[1];
-    // bilinear coefficients: v(ru, rv) = q0 + ru q1 + rv (q2 + ru q3)
-    const float q0 = (float)a, q1 = (float)(b - a), q2 = (float)(cc - a), q3 = (float)(d - cc - b + a);
-    const size_t idx = ((size_t)j * nr + r) * nc + c;
-    if (DT == MCT_F32) {
-        reinterpret_cast<float4 *>(out)[idx] = make_float4(q0, q1, q2, q3);
-    } else {
-        ushort4 h;
-        h.x = to_half_bits<DT>(q0);
-        h.y = to_half_bits<DT>(q1);
-        h.z = to_half_bits<DT>(q2);
-        h.w = to_half_bits<DT>(q3);
-        reinterpret_cast<ushort4 *>(out)[idx] = h;
+    const int re = rb + kQuadRows < nr ? rb + kQuadRows : nr;
+    const int64_t rstride = (int64_t)views * nu;
+    const float *p = in + ((int64_t)rb * views + j) * nu + c;
+    double a = p[0], b = p[1];
+    for (int r = rb; r < re; ++r) {
+        p += rstride;
+        const double cc = p[0], d = p[1];
+        // bilinear coefficients: v(ru, rv) = q0 + ru q1 + rv (q2 + ru q3)
+        const float q0 = (float)a, q1 = (float)(b - a), q2 = (float)(cc - a), q3 = (float)(d - cc - b + a);
+        const size_t idx = ((size_t)j * nr + r) * nc + c;
+        if (DT == MCT_F32) {
+            reinterpret_cast<float4 *>(out)[idx] = make_float4(q0, q1, q2, q3);
+        } else {
+            ushort4 h;
+            h.x = to_half_bits<DT>(q0);
+            h.y = to_half_bits<DT>(q1);
+            h.z = to_half_bits<DT>(q2);
+            h.w = to_half_bits<DT>(q3);
+            reinterpret_cast<ushort4 *>(out)[idx] = h;
+        }
+        a = cc;
+        b = d;
     }
 }
 
@@ -175,7 +211,7 @@ int mct_weight_views(mct_plan *plan, float *sgm, int32_t view_begin, int32_t vie
     BHCoef c = {};
     for (int t = 0; t < bh_order; ++t) c.c[t] = bh_coef[t];
     MCT_CHECK(g.n_v <= 65535 && view_count <= 65535, "rows and views must be <= 65535");
-    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for(g.n_u, 256), view_count, g.n_v), dim3(256), 0,
+    hipLaunchKernelGGL(weight_kernel, dim3(mct::blocks_for((g.n_u + 3) / 4, 256), view_count, g.n_v), dim3(256), 0,
                        ctx->stream,
                        sgm, g.n_v, view_count, g.n_u, plan->u, plan->v, plan->angles, g.sdd,
                        g.total_scan_angle, g.short_scan, g.curved, c, bh_order, view_begin);
@@ -228,7 +264,7 @@ int mct_pack_sinogram(mct_ctx *ctx, const float *in, int32_t rows, int32_t views
     MCT_CHECK(rows <= 65535 && views <= 65535, "rows and views must be <= 65535");
     dim3 grid;
     if (layout == MCT_QUAD)
-        grid = dim3(mct::blocks_for(n_u - 1, 256), rows - 1, views);
+        grid = dim3(mct::blocks_for(n_u - 1, 256), views, (rows - 1 + kQuadRows - 1) / kQuadRows);
     else if (layout == MCT_PAIR)
         grid = dim3(mct::blocks_for(n_u - 1, 256), rows, views);
     int64_t n = (int64_t)rows * views * n_u;
