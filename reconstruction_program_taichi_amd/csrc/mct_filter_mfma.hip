@@ -36,8 +36,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TM = 128, TN = 128, TK = 32;
 constexpr int APAD = 40;      // LDS row stride of an A tile, in BF16 elements (80 B)
-constexpr int KC = 256;       // K samples covered by one staged tap window
-constexpr int WLC = KC + TN;  // entries per shifted copy: fragments start below KC - TK + TN + 24
+constexpr int KC = 256;  // K samples covered by one staged tap window
+// Entries per shifted copy: fragments start below KC - TK + TN + 24, plus 16
+// entries of padding: a copy then spans 8 banks mod 64 more than a multiple
+// of 64, so the 16 lanes of a fragment read (two groups of 8 copies, the
+// groups 4 banks apart) land on 64 distinct banks.
+constexpr int WLC = KC + TN + 16;
 
 __device__ __forceinline__ __bf16 to_bf16(float f) {
     return __builtin_bit_cast(__bf16, mct::f32_to_bf16_rn(f));
