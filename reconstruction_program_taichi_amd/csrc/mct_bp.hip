@@ -366,12 +366,23 @@ __device__ __forceinline__ FastView column_view_fast(const BpArgs &a, float x, f
         w = rL * rL;
     }
     const float uidx = (u - a.u0) * a.inv_du;
-    int uf = (int)uidx;
-    uf = uf < 0 ? 0 : (uf > a.nu - 2 ? a.nu - 2 : uf);
-    cv.uf = (uint32_t)uf;
-    cv.ru = uidx - (float)uf;
+    // A column the staged kernel interpolates passes every view's u test, so
+    // 0 <= uidx <= nu - 2: truncation is the floor and fract(uidx) equals
+    // uidx - floor(uidx).  Other columns only need an address, which the
+    // window lookup clamps.
+    cv.uf = (uint32_t)(int)uidx;
+    cv.ru = __builtin_amdgcn_fractf(uidx);
     cv.coef = w * t.w;  // t.w = sid * dbeta * div_factor
     return cv;
+}
+
+// float(v) for 0 <= v < 2^24 from integer operations only, so a wave-uniform
+// v stays on the scalar unit (which has no float conversion).
+__device__ __forceinline__ float small_int_to_float(int v) {
+    if (v <= 0) return 0.f;
+    const int e = 31 - __builtin_clz((unsigned)v);  // 2^e <= v < 2^(e+1), e <= 23
+    const unsigned m = ((unsigned)v << (23 - e)) & 0x7fffffu;
+    return __uint_as_float(((unsigned)(127 + e) << 23) | m);
 }
 
 // d = fl(fl(mag * z) - v[0]) of slice iz, the quantity of the exact v test.
@@ -597,13 +608,13 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
                 // row coordinate relative to the window's first row; the
                 // subtraction is exact when 0 <= r0 <= t0, which holds
                 // whenever a slice of the chunk passes the range tests
-                const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - (float)wc.r0;
+                const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - small_int_to_float(wc.r0);
                 const float dt = cv.mag * zstep_t;
-                int lc = (int)cv.uf - wc.c0;
                 const int wmax = wc.w > 0 ? wc.w - 1 : 0;
-                lc = lc < 0 ? 0 : (lc > wmax ? wmax : lc);
+                const int lc = min(max((int)cv.uf - (int)wc.c0, 0), wmax);
                 // largest float below h: a clamped row coordinate floors to <= h - 1
-                const float tlim = wc.h > 0 ? __uint_as_float(__float_as_uint((float)wc.h) - 1u) : 0.f;
+                const float tlim =
+                    wc.h > 0 ? __uint_as_float(__float_as_uint(small_int_to_float(wc.h)) - 1u) : 0.f;
                 const unsigned char *src =
                     smem + (uint32_t)((j & 1) * cells + lc) * (uint32_t)sizeof(Raw);
                 // reads in batches of 8, all in flight before the first is used
