@@ -9,6 +9,9 @@
   ir:    IRN-CGLS-TV on 64 fan slices of 512 x 512 (geometry of fbp2d with
          120 views), 5 reweighting passes x 20 CG iterations, Lambda 1e-3,
          BetaTV 1e-6, slices coupled through TV as the reference does.
+  filter: the ramp filter alone on the rows of the cone FDK sinogram
+         (256 x 1080 rows of 1024 samples, Hamming t = 1 taps at 0.4 mm),
+         seeded uniform input, with --filter's method.
 Each returns a dict (printed as one JSON line by bench.py).
 """
 from __future__ import annotations
@@ -132,5 +135,32 @@ def ir(args) -> dict:
             "fp_bp_pairs": irr.num_iter_runned, "loss_first_last": [float(irr.loss[0]), float(irr.loss[-1])]}
 
 
+def filter_rows(args) -> dict:
+    from . import geometry as geo
+    from .device import Context
+
+    dev = torch.device("cuda", 0)
+    ctx = Context(0)
+    rows, n = 256 * 1080, 1024
+    gen = torch.Generator(device=dev).manual_seed(3)
+    x = torch.rand((rows, n), generator=gen, device=dev)
+    out = torch.empty_like(x)
+    taps = geo.hamming_taps(n, 0.4, 1.0, 1250.0)
+    method = args.filter
+
+    def step():
+        ctx.filter_rows(x, out, taps, 0.4, method)
+
+    step()
+    t = _sync_time(step, max(args.steps, 1))
+    products = {"f32": 1, "bf16": 1, "bf16x3": 3}[method]
+    flops = 2.0 * rows * n * n * products
+    line = {"workload": f"ramp filter, {rows} rows x {n} samples", "method": method,
+            "ms": round(t * 1e3, 4)}
+    if method != "f32":
+        line.update({"tflops": round(flops / t / 1e12, 1), "mfma_peak_frac": round(flops / t / 2.5e15, 4)})
+    return line
+
+
 def run(name: str, args) -> dict:
-    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir}[name](args)
+    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir, "filter": filter_rows}[name](args)
