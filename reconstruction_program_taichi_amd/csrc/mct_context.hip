@@ -20,7 +20,7 @@ int upload(T **dst, const T *src, size_t n) {
 
 void free_plan_arrays(mct_plan *p) {
     void *ptrs[] = {p->angles,   p->u,        p->v,       p->pmatrix, p->center_z, p->matrix_a,
-                    p->x_s,      p->bp_view,  p->fp_view, p->bp_iv,   p->bp_win};
+                    p->x_s,      p->bp_view,  p->fp_view, p->bp_iv,   p->bp_win,   p->fp_ltab};
     for (void *q : ptrs)
         if (q) hipFree(q);
 }

@@ -20,6 +20,7 @@
 #include "mct_internal.h"
 
 #include <cmath>
+#include <vector>
 
 namespace {
 
@@ -292,10 +293,15 @@ __device__ __forceinline__ int wave_max(int v) {
 // record strides of x and y in it, pz the stride of z (cone) or the fan
 // slice's first record.  Every lane of the wavefront must call this (active =
 // false: a lane without a ray).  Samples go four at a time, loads first.
+//
+// The sample distance l of step k comes from ltab (the plan's table in the
+// same FP32 expression order); k is wave-uniform, so it is a scalar load.
+// A sample outside the volume reads cell 0 and adds 0; inside, the indices
+// are those of the reference's in-bounds test.  pz (cone) is below 2^24.
 template <bool CONE>
-__device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j,
-                             __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy, uint32_t pz,
-                             int seg, int nseg) {
+__device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bool active, int ui,
+                             int vi, int j, __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy,
+                             uint32_t pz, int seg, int nseg) {
     const Ray r = make_ray(a, active ? ui : 0, vi, j);
     const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
     int k_lo, k_hi;
@@ -324,34 +330,33 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
 #pragma unroll
         for (int i = 0; i < U; ++i) {
             const int k = k0 + i;
-            const float l = (float)k * a.step * a.diag + a.lmin;
+            const float l = ltab[k < a.count ? k : a.count - 1];
             const float xr = fmaf(l, r.bx, r.ax);
             const float yr = fmaf(l, r.by, r.ay);
             const float fx = (xr - a.x0) * a.inv_pix;
             const float fy = (yr - a.y0) * (-a.inv_pix);
             const float flx = floorf(fx), fly = floorf(fy);
-            int xi = (int)flx, yi = (int)fly;
-            ok[i] = active && k < k_hi && (uint32_t)xi < (uint32_t)nc && (uint32_t)yi < (uint32_t)nc;
+            const int xi = (int)flx, yi = (int)fly;
+            bool in = active && k < k_hi && (uint32_t)xi < (uint32_t)nc && (uint32_t)yi < (uint32_t)nc;
             xw[i] = fx - flx;
             yw[i] = fy - fly;
-            xi = xi < 0 ? 0 : (xi > nc - 1 ? nc - 1 : xi);
-            yi = yi < 0 ? 0 : (yi > nc - 1 ? nc - 1 : yi);
             uint32_t cell = __umul24((uint32_t)xi, sx) + __umul24((uint32_t)yi, sy);
             if (CONE) {
                 const float zr = fmaf(l, r.bz, r.az);
                 const float fz = (zr - z0) * a.inv_vh;
                 const float flz = floorf(fz);
-                int zi = (int)flz;
-                ok[i] = ok[i] && (uint32_t)zi < (uint32_t)(a.nz - 1);
+                const int zi = (int)flz;
+                in = in && (uint32_t)zi < (uint32_t)(a.nz - 1);
                 zw[i] = fz - flz;
-                zi = zi < 0 ? 0 : (zi > a.nz - 2 ? a.nz - 2 : zi);
-                cell += (uint32_t)zi * pz;
+                cell += __umul24((uint32_t)zi, pz);
+                cell = in ? cell : 0u;
                 q[i][0] = load_cell(rsrc, cell * kRec);
                 q[i][CONE ? 1 : 0] = load_cell(rsrc, cell * kRec + 16u);
             } else {
-                cell += pz;
+                cell = in ? cell + pz : 0u;
                 q[i][0] = load_cell(rsrc, cell * kRec);
             }
+            ok[i] = in;
         }
 #pragma unroll
         for (int i = 0; i < U; ++i) {
@@ -376,7 +381,8 @@ __device__ float trace_cells(const FpArgs &a, bool active, int ui, int vi, int j
 // nwaves counts the numbered wavefronts (idle ones of a partial band
 // included); each segment of a split ray range repeats the numbering.
 template <bool CONE>
-__global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *xm, const float4 *ym,
+__global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float *__restrict__ ltab,
+                                                       const float4 *xm, const float4 *ym,
                                                        uint32_t bytes, int nub, int nwaves, int nseg,
                                                        int rows4, float *partial) {
     // wave-uniform by construction; readfirstlane keeps the decode below and
@@ -420,7 +426,7 @@ __global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float4 *x
         __builtin_amdgcn_make_buffer_rsrc((void *)(ymaj ? ym : xm), 0, (int)bytes, kBufferDword3);
     float acc = 0.f;
     for (int i = 0; i < a.os; ++i)
-        acc += trace_cells<CONE>(a, active, uo * a.os + i, vi, j, rsrc, sx, sy, pz, seg, nseg);
+        acc += trace_cells<CONE>(a, ltab, active, uo * a.os + i, vi, j, rsrc, sx, sy, pz, seg, nseg);
     if (!inside) return;
     const float v = masked ? 0.f : acc / (float)a.os;
     if (nseg == 1)
@@ -600,6 +606,34 @@ int fill_args(mct_plan *p, FpArgs &a, int count_mode) {
     return MCT_OK;
 }
 
+// The plan's table of sample distances l[k] = k * step * diag + l_min for
+// k < count, in the FP32 expression order of the direct kernel's loop
+// (rebuilt when the scalars change; one synchronisation when built).
+int ensure_step_table(mct_plan *p, const FpArgs &a) {
+    if (a.count <= 0) return MCT_OK;
+    if (p->fp_ltab && p->fp_ltab_n == a.count && p->fp_ltab_key[0] == a.step &&
+        p->fp_ltab_key[1] == a.diag && p->fp_ltab_key[2] == a.lmin)
+        return MCT_OK;
+    mct_ctx *ctx = p->ctx;
+    if (p->fp_ltab) {
+        MCT_HIP(hipStreamSynchronize(ctx->stream));
+        MCT_HIP(hipFree(p->fp_ltab));
+        p->fp_ltab = nullptr;
+        p->fp_ltab_n = 0;
+    }
+    std::vector<float> l((size_t)a.count);
+    for (int k = 0; k < a.count; ++k) l[(size_t)k] = (float)k * a.step * a.diag + a.lmin;
+    MCT_HIP(hipMalloc((void **)&p->fp_ltab, sizeof(float) * l.size()));
+    MCT_HIP(hipMemcpyAsync(p->fp_ltab, l.data(), sizeof(float) * l.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    MCT_HIP(hipStreamSynchronize(ctx->stream));  // l is released on return
+    p->fp_ltab_n = a.count;
+    p->fp_ltab_key[0] = a.step;
+    p->fp_ltab_key[1] = a.diag;
+    p->fp_ltab_key[2] = a.lmin;
+    return MCT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -647,10 +681,13 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         nseg = nseg < 1 ? 1 : (nseg > 16 ? 16 : nseg);
         const int64_t nblocks = (nwaves * nseg + 3) / 4;
         // 32-bit record offsets; gridDim.z of the pack launch; wavefront index
+        // (and 24-bit plane strides for the cone kernel's address arithmetic)
         const bool fits = nc >= 1 && planes >= 1 && bytes < ((int64_t)1 << 32) && planes <= 65535 &&
-                          nwaves * nseg < ((int64_t)1 << 31);
+                          nwaves * nseg < ((int64_t)1 << 31) && nc * nc < ((int64_t)1 << 24);
         if (fits) {
-            int rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, 2 * (bytes / 4));
+            int rc = ensure_step_table(plan, a);
+            if (rc) return rc;
+            rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, 2 * (bytes / 4));
             if (rc) return rc;
             const int64_t nout = (int64_t)a.nrows * a.nviews * a.nu_out;
             if (nseg > 1) {
@@ -664,13 +701,13 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
                 hipLaunchKernelGGL(fp_pack_kernel<true>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
+                                   ctx->stream, a, plan->fp_ltab, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
                                    (int)nseg, rows4, ctx->fp_partial);
             } else {
                 hipLaunchKernelGGL(fp_pack_kernel<false>, pgrid, dim3(256), 0, ctx->stream, vol,
                                    g.img_dim, xm, ym);
                 hipLaunchKernelGGL(fp_cells_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
+                                   ctx->stream, a, plan->fp_ltab, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
                                    (int)nseg, rows4, ctx->fp_partial);
             }
             if (nseg > 1)

@@ -80,6 +80,11 @@ struct mct_plan {
     float cos_a0 = 1.f, sin_a0 = 0.f;  // of angles[0] (P-matrix forward projection)
     float zpv = 0.f;                   // helical z shift per view (forward projection)
     int fp_count_mode = 0;
+    // Sample distances of the cell forward projector, l[k] = k * step * diag
+    // + l_min in FP32 (built on its first use) and the scalars they came from
+    float *fp_ltab = nullptr;
+    int fp_ltab_n = 0;
+    float fp_ltab_key[3] = {0.f, 0.f, 0.f};
     // Exact detector-range tests of run_mgfbp.py:1142-1144 and 1170-1173 as
     // intervals: the u test passes iff ua <= u <= ub for u = fl(mag * yr);
     // the v test passes iff da <= d <= db for d = fl(fl(mag * z) - v[0]).
