@@ -857,7 +857,9 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
 // leaves the detector in any view is 0 (truncation flag); v off the detector
 // adds 0.  PLAIN FP32 sinogram [rows][views][nu]; optional mask multiplies
 // each tap.  ZC slices per thread.
-template <int ZC>
+// FLATZ: fan beam with voxel height 0, where every slice sits at z = the
+// view's centre, so the ray-length weight is one value per (column, view).
+template <int ZC, bool FLATZ>
 __global__ __launch_bounds__(256) void bp_ir_axial_kernel(BpArgs a) {
     int ix, iy, zci;
     decode_tile(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd), ix, iy, zci);
@@ -897,11 +899,13 @@ __global__ __launch_bounds__(256) void bp_ir_axial_kernel(BpArgs a) {
         uf = uf > a.nu - 2 ? a.nu - 2 : uf;
         float ru = uidx - (float)uf;
         const size_t vb = (size_t)j * a.nu + uf;
+        // (zb[k] = +-0 when FLATZ: z = czv exactly in every slice)
+        const float wgt_view = FLATZ ? sqrtf(dxs * dxs + dys * dys + czv * czv) / den : 0.f;
 #pragma unroll
         for (int k = 0; k < ZC; ++k) {
             if (k >= nk) continue;
             float z = zb[k] + czv;
-            float wgt = sqrtf(dxs * dxs + dys * dys + z * z) / den;
+            float wgt = FLATZ ? wgt_view : sqrtf(dxs * dxs + dys * dys + z * z) / den;
             float val;
             if (a.cone) {
                 float vidx = (mag * z - a.v0) / a.det_h * a.adv / a.dv;
@@ -1412,7 +1416,10 @@ int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, 
     a.tiles_y = a.tiles_x;
     a.nzc = (g.img_dim_z + zc - 1) / zc;
     int nblocks = a.tiles_x * a.tiles_y * a.nzc;
-    hipLaunchKernelGGL(bp_ir_axial_kernel<8>, dim3(nblocks), dim3(256), 0, ctx->stream, a);
+    if (!g.cone && g.vox_h == 0.f)
+        hipLaunchKernelGGL((bp_ir_axial_kernel<8, true>), dim3(nblocks), dim3(256), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((bp_ir_axial_kernel<8, false>), dim3(nblocks), dim3(256), 0, ctx->stream, a);
     MCT_LAUNCHED();
     return MCT_OK;
 }
