@@ -15,13 +15,16 @@ iterate state on the device and overlaps the work of different requests:
     in its own launch (its tiles fill the device) and all fan slices in ONE
     persistent launch (mct_backproject_batch), on compute stream A;
   * CGLS-TV jobs run their iterations on their own compute stream (a pool of
-    several, round robin, is available), overlapping A;
+    several, round robin, is available), overlapping A; jobs that arrive
+    while it is busy wait and then go as stacks of same-geometry jobs, one
+    forward and one back-projection per iteration for the whole stack;
   * results go device -> host on the copy stream; buffers return to the
     budget when the copy has finished.
 
-Each job's result is bit-identical alone (Server.run_plain) and in the mix:
-the batched launch runs the same tile code as the single-job kernels, and the
-CG reductions are deterministic.
+Each job's result is bit-identical alone (run_plain) and in the mix: the
+batched launch runs the same tile code as the single-job kernels, a CGLS
+stack treats its slices independently with per-job scalars, and the CG
+reductions are deterministic.
 """
 from __future__ import annotations
 
@@ -152,6 +155,63 @@ def cgls_run(ctx: Context, plan: Plan, job: Job, sgm: torch.Tensor, x: torch.Ten
     return x
 
 
+def cgls_stack_spec(spec: geo.GeometrySpec, n: int) -> geo.GeometrySpec:
+    """The IR geometry of n fan CGLS jobs of one geometry stacked as the
+    slices of one problem (one detector row per slice)."""
+    return replace(spec, img_dim_z=n, v=geo.det_pos_array(n, n, 0, 0), n_v_full=n)
+
+
+def cgls_run_stack(ctx: Context, plan: Plan, jobs: list[Job], sgm: torch.Tensor,
+                   x: torch.Tensor) -> torch.Tensor:
+    """cgls_run for n fan jobs of one geometry (plan: cgls_stack_spec; sgm
+    (n, views, n_u); x (n, N, N)): one forward and one back-projection per
+    iteration for all of them, the TV term slice by slice (no coupling
+    between jobs), and the inner products, step scalars and updates per job.
+    Each job's iterates equal its own cgls_run bit for bit: every kernel
+    treats the slices independently, and the forward projector splits each
+    ray's steps as it would for one slice (split target scaled by n)."""
+    n = len(jobs)
+    job = jobs[0]
+    s = job.spec
+    dev = sgm.device
+    vshape = (n, s.img_dim, s.img_dim)
+    B = {k: torch.empty(vshape, dtype=torch.float32, device=dev) for k in ("bpb", "d", "r", "q", "t", "w")}
+    fp = torch.empty_like(sgm)
+    mask = torch.empty_like(sgm)
+    sc = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    c1 = job.lam * (s.n_v * s.n_views * s.n_u) / (s.img_dim_z * s.img_dim * s.img_dim)
+    ctx.set_fp_split(256 * n)
+    try:
+        plan.ray_mask(mask)
+        plan.backproject_ir(sgm, mask, B["bpb"])
+        x.zero_()
+        B["w"].fill_(1.0)
+
+        def apply_A(v, out):
+            plan.forward_project(v, fp, mask, 1)
+            plan.backproject_ir(fp, None, B["t"])
+            ctx.dtwd(v, B["w"], c1, B["t"], out, couple_z=False)
+
+        apply_A(x, B["q"])
+        ctx.axpby(1.0, B["bpb"], -1.0, B["q"], B["d"])
+        B["r"].copy_(B["d"])
+        for _ in range(job.iters):
+            apply_A(B["d"], B["q"])
+            for i in range(n):
+                ctx.dot(B["r"][i], B["r"][i], sc[i, 0:1])
+                ctx.dot(B["d"][i], B["q"][i], sc[i, 1:2])
+                ctx.cg_update(x[i], B["r"][i], B["d"][i], B["q"][i], sc[i])
+                ctx.cg_direction(B["d"][i], B["r"][i], sc[i])
+    finally:
+        ctx.set_fp_split(0)
+    return x
+
+
+def cgls_stack_key(job: Job) -> tuple:
+    """Jobs that can share a stacked CGLS run."""
+    return (geometry_key(job.spec), job.iters, job.lam)
+
+
 def run_plain(jobs: list[Job], device: int = 0) -> dict:
     """One job at a time, synchronously, in arrival order: fresh plan,
     allocation, taps upload and host <-> device copies paid per job (the
@@ -199,7 +259,8 @@ class Server:
     """Continuous-batching server on one device (see the module comment)."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, max_batch: int = 64,
-                 cgls_streams: int = 1, h2d_window: int = 1 << 40, small_bytes: int = 0):
+                 cgls_streams: int = 1, h2d_window: int = 1 << 40, small_bytes: int = 0,
+                 max_cgls_stack: int = 16):
         self.device = device
         self.dev = torch.device("cuda", device)
         self.s_copy = torch.cuda.Stream(self.dev)  # host <-> device copies
@@ -217,6 +278,13 @@ class Server:
             c.bind_stream(s)
         self.plans_b = [PlanCache(c) for c in self.ctx_b]
         self._next_b = 0
+        # CGLS jobs that are loaded wait per stream until it is idle, then go
+        # as stacks of same-geometry jobs (one forward and back-projection
+        # per iteration for the whole stack)
+        self.max_cgls_stack = max(1, int(max_cgls_stack))
+        self._cgls_wait: list[list[Job]] = [[] for _ in self.s_b]
+        self._cgls_busy: list = [None for _ in self.s_b]
+        self.cgls_stacks: list[int] = []  # size of every CGLS run launched
         self.budget = int(hbm_budget)
         self.used = 0
         self.max_batch = max_batch
@@ -276,18 +344,43 @@ class Server:
             done.record(s)
         self._copy_back(batch, done)
 
-    def _launch_cgls(self, job: Job):
-        lane = job._dev["lane"]
+    def _launch_cgls(self, jobs: list[Job], lane: int) -> torch.cuda.Event:
+        """One stacked CGLS run of same-geometry jobs on CGLS stream `lane`."""
         s = self.s_b[lane]
+        n = len(jobs)
+        self.cgls_stacks.append(n)
         with torch.cuda.stream(s):
-            s.wait_event(job._events["loaded"])
-            plan = self.plans_b[lane].get(job.spec)
-            x = torch.empty(job.vol_shape(), dtype=torch.float32, device=self.dev)
-            cgls_run(self.ctx_b[lane], plan, job, job._dev["sgm"], x)
-            job._dev["vol"] = x
+            for job in jobs:
+                s.wait_event(job._events["loaded"])
+            spec = jobs[0].spec
+            plan = self.plans_b[lane].get(cgls_stack_spec(spec, n) if n > 1 else spec)
+            sgm = torch.cat([j._dev["sgm"] for j in jobs]) if n > 1 else jobs[0]._dev["sgm"]
+            x = torch.empty((n, spec.img_dim, spec.img_dim), dtype=torch.float32, device=self.dev)
+            if n > 1:
+                cgls_run_stack(self.ctx_b[lane], plan, jobs, sgm, x)
+            else:
+                cgls_run(self.ctx_b[lane], plan, jobs[0], sgm, x)
+            for i, job in enumerate(jobs):
+                job._dev["vol"] = x[i:i + 1]
             done = torch.cuda.Event()
             done.record(s)
-        self._copy_back([job], done)
+        self._copy_back(jobs, done)
+        return done
+
+    def _pump_cgls(self):
+        """Starts the waiting CGLS jobs of every idle CGLS stream, grouped
+        into stacks of one geometry."""
+        for lane, wait in enumerate(self._cgls_wait):
+            busy = self._cgls_busy[lane]
+            if not wait or (busy is not None and not busy.query()):
+                continue
+            groups: dict = {}
+            for job in wait:
+                groups.setdefault(cgls_stack_key(job), []).append(job)
+            wait.clear()
+            for group in groups.values():
+                for i in range(0, len(group), self.max_cgls_stack):
+                    self._cgls_busy[lane] = self._launch_cgls(group[i:i + self.max_cgls_stack], lane)
 
     def _copy_back(self, jobs: list[Job], computed: torch.cuda.Event):
         with torch.cuda.stream(self.s_copy):
@@ -339,9 +432,10 @@ class Server:
                     self._launch_fbp_batch(fb)
                 for j in ready:
                     if j.kind == "cgls":
-                        self._launch_cgls(j)
+                        self._cgls_wait[j._dev["lane"]].append(j)
                 running.extend(ready)
-            finished = [j for j in running if j._events["done"].query()]
+            self._pump_cgls()
+            finished = [j for j in running if "done" in j._events and j._events["done"].query()]
             for j in finished:
                 j.t_done = time.perf_counter() - t0
                 self.used -= j.nbytes()
