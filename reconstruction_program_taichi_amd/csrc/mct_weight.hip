@@ -105,19 +105,38 @@ __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int n
     }
 }
 
-// run_mgfbp.py:1019-1031 (vertical Gaussian, +-10 rows, ascending n).
+// run_mgfbp.py:1019-1031 (vertical Gaussian, +-10 rows, ascending n):
+// out[i][jk] = det_h * sum_{|i-n|<=10} in[n][jk] * kern[i + rows-1 - n].
+// A thread owns kVRows consecutive output rows of one (view, element) jk and
+// holds the kVRows + 20 input rows they read in registers, so every input
+// is read once per thread instead of once per output; each sum still runs
+// over n in ascending order.
+constexpr int kVRows = 16;
 __global__ void vertical_kernel(const float *__restrict__ in, float *__restrict__ out, int rows,
                                 int views, int nu, const float *__restrict__ kern, float det_h) {
-    int64_t plane = (int64_t)views * nu;
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= plane * rows) return;
-    int i = (int)(idx / plane);
-    int64_t jk = idx % plane;
-    int n0 = i - 10 < 0 ? 0 : i - 10;
-    int n1 = i + 10 > rows - 1 ? rows - 1 : i + 10;
-    float acc = 0.f;
-    for (int n = n0; n <= n1; ++n) acc += in[n * plane + jk] * kern[i + (rows - 1) - n];
-    out[idx] = acc * det_h;
+    const int64_t plane = (int64_t)views * nu;
+    const int64_t jk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (jk >= plane) return;
+    const int i0 = (int)blockIdx.y * kVRows;  // first output row of this thread
+    float v[kVRows + 20];                      // input rows i0 - 10 .. i0 + kVRows + 9
+#pragma unroll
+    for (int t = 0; t < kVRows + 20; ++t) {
+        const int n = i0 - 10 + t;
+        v[t] = (n >= 0 && n < rows) ? in[n * plane + jk] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < kVRows; ++r) {
+        const int i = i0 + r;
+        if (i >= rows) break;
+        float acc = 0.f;
+#pragma unroll
+        for (int d = -10; d <= 10; ++d) {
+            const int n = i + d;
+            if (n < 0 || n >= rows) continue;
+            acc += v[r + 10 + d] * kern[i + (rows - 1) - n];
+        }
+        out[i * plane + jk] = acc * det_h;
+    }
 }
 
 template <int DT> __device__ __forceinline__ uint16_t to_half_bits(float f) {
@@ -232,9 +251,11 @@ int mct_filter_vertical(mct_ctx *ctx, const float *in, float *out, int32_t rows,
     int rc = mct::upload_cached(ctx, &ctx->vkern_dev, &ctx->vkern_cap, &ctx->vkern_hash, kern,
                                 2 * (int64_t)rows - 1);
     if (rc) return rc;
-    int64_t n = (int64_t)rows * views * n_u;
-    hipLaunchKernelGGL(vertical_kernel, dim3(mct::blocks_for(n, 256)), dim3(256), 0, ctx->stream,
-                       in, out, rows, views, n_u, ctx->vkern_dev, det_h);
+    const int64_t plane = (int64_t)views * n_u;
+    MCT_CHECK((rows + kVRows - 1) / kVRows <= 65535, "too many rows (%d)", rows);
+    hipLaunchKernelGGL(vertical_kernel,
+                       dim3(mct::blocks_for(plane, 256), (unsigned)((rows + kVRows - 1) / kVRows)),
+                       dim3(256), 0, ctx->stream, in, out, rows, views, n_u, ctx->vkern_dev, det_h);
     MCT_LAUNCHED();
     return MCT_OK;
 }

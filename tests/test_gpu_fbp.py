@@ -91,7 +91,7 @@ def test_filter_rows_mfma(ctx, method, tol, n, rows):
     assert err <= tol
 
 
-@pytest.mark.parametrize("rows", [1, 7, 48])
+@pytest.mark.parametrize("rows", [1, 7, 37, 48])
 def test_filter_vertical_matches_oracle(ctx, rows):
     from reconstruction_program_taichi_amd import geometry as geo
 
