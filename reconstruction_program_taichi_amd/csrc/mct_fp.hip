@@ -267,6 +267,15 @@ __device__ __forceinline__ float cell_lerp(float4 q, float xw, float yw) {
     return fmaf(yw, fmaf(xw, q.w, q.z), fmaf(xw, q.y, q.x));
 }
 
+// (int)floor(x) in one instruction.  With fract(x), which equals x - floor(x)
+// exactly for x >= 0 (the only samples that count), it replaces a floor, a
+// convert and a subtraction per axis.
+__device__ __forceinline__ int floor_to_int(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 __device__ __forceinline__ float4 load_cell(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
@@ -296,8 +305,10 @@ __device__ __forceinline__ int wave_max(int v) {
 //
 // The sample distance l of step k comes from ltab (the plan's table in the
 // same FP32 expression order); k is wave-uniform, so it is a scalar load.
-// A sample outside the volume reads cell 0 and adds 0; inside, the indices
-// are those of the reference's in-bounds test.  pz (cone) is below 2^24.
+// A sample outside the volume loads from past the end of rsrc, which returns
+// zeros, so it adds +0; inside, the indices are those of the reference's
+// in-bounds test.  pz (cone) is below 2^24; the fan's record offsets are
+// formed in bytes (two 24-bit multiply-adds).
 template <bool CONE>
 __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bool active, int ui,
                              int vi, int j, __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy,
@@ -321,12 +332,14 @@ __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bo
     const float dl = a.step * a.diag;
     const int nc = a.nx - 1;
     constexpr uint32_t kRec = CONE ? 32u : 16u;
+    // byte offset past every record (the orders hold under 4 GiB, a multiple of kRec)
+    constexpr uint32_t kOut = 0u - kRec;
+    const uint32_t bsx = sx * kRec, bsy = sy * kRec, bpz = pz * kRec;
     float sum = 0.f;
     constexpr int U = 4;
     for (int k0 = k_lo; k0 < k_hi; k0 += U) {
         float4 q[U][CONE ? 2 : 1];
         float xw[U], yw[U], zw[U];
-        bool ok[U];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
             const int k = k0 + i;
@@ -335,29 +348,27 @@ __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bo
             const float yr = fmaf(l, r.by, r.ay);
             const float fx = (xr - a.x0) * a.inv_pix;
             const float fy = (yr - a.y0) * (-a.inv_pix);
-            const float flx = floorf(fx), fly = floorf(fy);
-            const int xi = (int)flx, yi = (int)fly;
+            const int xi = floor_to_int(fx), yi = floor_to_int(fy);
             bool in = active && k < k_hi && (uint32_t)xi < (uint32_t)nc && (uint32_t)yi < (uint32_t)nc;
-            xw[i] = fx - flx;
-            yw[i] = fy - fly;
-            uint32_t cell = __umul24((uint32_t)xi, sx) + __umul24((uint32_t)yi, sy);
+            xw[i] = __builtin_amdgcn_fractf(fx);
+            yw[i] = __builtin_amdgcn_fractf(fy);
             if (CONE) {
                 const float zr = fmaf(l, r.bz, r.az);
                 const float fz = (zr - z0) * a.inv_vh;
-                const float flz = floorf(fz);
-                const int zi = (int)flz;
+                const int zi = floor_to_int(fz);
                 in = in && (uint32_t)zi < (uint32_t)(a.nz - 1);
-                zw[i] = fz - flz;
-                cell += __umul24((uint32_t)zi, pz);
-                cell = in ? cell : 0u;
-                q[i][0] = load_cell(rsrc, cell * kRec);
-                q[i][CONE ? 1 : 0] = load_cell(rsrc, cell * kRec + 16u);
+                zw[i] = __builtin_amdgcn_fractf(fz);
+                const uint32_t cell =
+                    __umul24((uint32_t)xi, sx) + __umul24((uint32_t)yi, sy) + __umul24((uint32_t)zi, pz);
+                const uint32_t off = in ? cell * kRec : kOut;
+                q[i][0] = load_cell(rsrc, off);
+                q[i][CONE ? 1 : 0] = load_cell(rsrc, off + 16u);  // kOut + 16 is past the end too
             } else {
-                cell = in ? cell + pz : 0u;
-                q[i][0] = load_cell(rsrc, cell * kRec);
+                const uint32_t off = __umul24((uint32_t)xi, bsx) + (__umul24((uint32_t)yi, bsy) + bpz);
+                q[i][0] = load_cell(rsrc, in ? off : kOut);
             }
-            ok[i] = in;
         }
+        // an outside sample's cell is zeros and its weights lie in [0, 1): it adds +0
 #pragma unroll
         for (int i = 0; i < U; ++i) {
             float val = cell_lerp(q[i][0], xw[i], yw[i]);
@@ -365,7 +376,7 @@ __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bo
                 const float hi = cell_lerp(q[i][CONE ? 1 : 0], xw[i], yw[i]);
                 val = fmaf(zw[i], hi - val, val);
             }
-            sum += ok[i] ? val : 0.f;
+            sum += val;
         }
     }
     return sum * dl;
