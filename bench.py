@@ -181,7 +181,8 @@ def main() -> int:
 
     plan.backproject_fbp = timed_bp
 
-    # Weighting runs in place, so every timed step gets a resident copy of the
+    # Weighting may run in place (FP32 filter, vertical Gaussian; the MFMA
+    # filters weigh in their input pass), so every timed step gets a resident copy of the
     # sinogram of its own (made before the timed region); past
     # RESIDENT_INPUT_BYTES the copies are replaced by a device copy of the
     # input inside each step.

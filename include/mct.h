@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 5
+#define MCT_ABI_VERSION 6
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -168,6 +168,19 @@ int mct_filter_vertical(mct_ctx *ctx, const float *in, float *out, int32_t rows,
  * calls on one context must be ordered, as calls on its stream are. */
 int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, int32_t n,
                     const float *taps, float scale, int32_t method);
+
+/* BHCorrection + WeightSgm (run_mgfbp.py:946-1011) and the ramp convolution
+ * (run_mgfbp.py:1033-1046) of views [view_begin, view_begin + view_count) in
+ * one pass over the sinogram: the same as mct_weight_views on sgm followed
+ * by mct_filter_rows(plan's context, sgm, out, n_v * view_count, n_u, ...),
+ * bit for bit, except that sgm is left unweighted (the weighting is applied
+ * where the MFMA methods convert their input rows to BF16 planes).
+ * sgm DEVICE FP32 [n_v][view_count][n_u]; out DEVICE FP32, same shape;
+ * method: MCT_FILTER_BF16_MFMA or MCT_FILTER_BF16X3_MFMA only.  There is no
+ * vertical Gaussian between the two steps here. */
+int mct_weight_filter_rows(mct_plan *plan, const float *sgm, float *out, int32_t view_begin,
+                           int32_t view_count, const float *bh_coef, int32_t bh_order,
+                           const float *taps, float scale, int32_t method);
 
 /* Repack a DEVICE FP32 filtered sinogram [rows][views][n_u] into the dtype and
  * layout the back-projector reads.  Replaces the reference's Taichi field

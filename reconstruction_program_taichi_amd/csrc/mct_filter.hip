@@ -9,6 +9,7 @@
 // consecutive outputs; their taps form a window that slides down by one tap
 // per m, so each m costs one new tap read from LDS and 16 FMAs.
 #include "mct_internal.h"
+#include "mct_weigh.h"
 
 namespace {
 
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void ramp_direct_kernel(const float *__restric
 
 namespace mct {
 int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, int n,
-                     const float *taps, float scale, int method);
+                     const float *taps, float scale, int method, const WeighSpec *w);
 }
 
 extern "C" {
@@ -91,7 +92,7 @@ int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, i
     MCT_CHECK(in && out, "NULL argument");
     MCT_CHECK(in != out, "the row filter cannot run in place");
     if (method != MCT_FILTER_F32_DIRECT)
-        return mct::filter_rows_mfma(ctx, in, out, n_rows, n, taps, scale, method);
+        return mct::filter_rows_mfma(ctx, in, out, n_rows, n, taps, scale, method, nullptr);
     int rc = mct::upload_cached(ctx, &ctx->taps_dev, &ctx->taps_cap, &ctx->taps_hash, taps,
                                 2 * (int64_t)n - 1);
     if (rc) return rc;
@@ -105,6 +106,36 @@ int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, i
                        in, out, n_rows, n, ctx->taps_dev, scale);
     MCT_LAUNCHED();
     return MCT_OK;
+}
+
+int mct_weight_filter_rows(mct_plan *plan, const float *sgm, float *out, int32_t view_begin,
+                           int32_t view_count, const float *bh_coef, int32_t bh_order,
+                           const float *taps, float scale, int32_t method) {
+    MCT_CHECK(plan && sgm && out && taps, "NULL argument");
+    MCT_CHECK(sgm != out, "the row filter cannot run in place");
+    MCT_CHECK(method == MCT_FILTER_BF16_MFMA || method == MCT_FILTER_BF16X3_MFMA,
+              "weighting is fused only into the MFMA filter methods (got %d)", method);
+    MCT_CHECK(bh_order >= 0 && bh_order <= mct::kMaxBH, "bh_order must be in [0, %d]", mct::kMaxBH);
+    MCT_CHECK(bh_order == 0 || bh_coef, "bh_coef is NULL");
+    const mct_geometry &g = plan->g;
+    MCT_CHECK(view_begin >= 0 && view_count >= 1 && view_count <= g.n_views - view_begin,
+              "views [%d, %d + %d) outside the scan's %d views", view_begin, view_begin, view_count,
+              g.n_views);
+    MCT_CHECK(g.n_u >= 1 && g.n_u <= 4096, "row length must be in [1, 4096] (got %d)", g.n_u);
+    mct::WeighSpec w = {};
+    w.u = plan->u;
+    w.v = plan->v;
+    w.ang = plan->angles;
+    w.sdd = g.sdd;
+    w.scan = g.total_scan_angle;
+    w.short_scan = g.short_scan;
+    w.curved = g.curved;
+    for (int t = 0; t < bh_order; ++t) w.bh.c[t] = bh_coef[t];
+    w.bh_order = bh_order;
+    w.nviews = view_count;
+    w.view0 = view_begin;
+    return mct::filter_rows_mfma(plan->ctx, sgm, out, (int64_t)g.n_v * view_count, g.n_u, taps, scale,
+                                 method, &w);
 }
 
 }  // extern "C"

@@ -5,7 +5,10 @@
 // Two launches:
 //   * split_bf16_kernel converts every input row once into BF16 planes
 //     [rows][npad]: hi = bf16(x) and, for the split method, lo = bf16(x - hi),
-//     zero past n;
+//     zero past n; with a WeighSpec it first weighs each sample exactly as
+//     the weighting kernel does (same FP32 operations), so weighting and
+//     filtering read the sinogram once and the planes are those of the
+//     weighted sinogram;
 //   * ramp_gemm_kernel: workgroup tile 128 rows x 128 output columns, four
 //     wavefronts of 64 x 64, each a 4 x 4 grid of v_mfma_f32_16x16x32_bf16
 //     accumulators (FP32).  Per K step of 32 the A tile (128 rows x 32
@@ -24,6 +27,7 @@
 // B[8(l/16) .. +7][l % 16]; the 4 accumulators of lane l are
 // D[4(l/16) .. +3][l % 16].
 #include "mct_internal.h"
+#include "mct_weigh.h"
 
 #include <cmath>
 #include <cstring>
@@ -47,11 +51,12 @@ __device__ __forceinline__ __bf16 to_bf16(float f) {
     return __builtin_bit_cast(__bf16, mct::f32_to_bf16_rn(f));
 }
 
-// One thread per 8 samples of a row: in[r][m0 .. m0+7] -> hi/lo planes.
-template <bool SPLIT>
+// One thread per 8 samples of a row: in[r][m0 .. m0+7] -> hi/lo planes
+// (WEIGH: row r is (detector row r / w.nviews, view r % w.nviews of the block)).
+template <bool SPLIT, bool WEIGH>
 __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict__ in, int64_t rows,
                                                          int n, int npad, __bf16 *__restrict__ hi,
-                                                         __bf16 *__restrict__ lo) {
+                                                         __bf16 *__restrict__ lo, mct::WeighSpec w) {
     const int groups = npad / 8;
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= rows * groups) return;
@@ -67,6 +72,14 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict
     } else {
 #pragma unroll
         for (int q = 0; q < 8; ++q) x[q] = m0 + q < n ? src[q] : 0.f;
+    }
+    if (WEIGH) {
+        const int s = (int)(r / w.nviews), i = (int)(r - (int64_t)s * w.nviews);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (m0 + q < n)
+                x[q] = mct::weigh(x[q], m0 + q, s, w.view0 + i, w.u, w.v, w.ang, w.sdd, w.scan,
+                                  w.short_scan, w.curved, w.bh, w.bh_order);
     }
     bf16x8 h, l;
 #pragma unroll
@@ -216,8 +229,10 @@ using mct::fnv1a;
 
 namespace mct {
 
+// w != NULL: the input rows are an unweighted [n_v][w->nviews][n] block,
+// weighed in the split pass (n_rows = n_v * w->nviews).
 int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, int n,
-                     const float *taps, float scale, int method) {
+                     const float *taps, float scale, int method, const WeighSpec *w) {
     MCT_CHECK(method == MCT_FILTER_BF16_MFMA || method == MCT_FILTER_BF16X3_MFMA,
               "unknown filter method %d", method);
     const int ntaps = 2 * n - 1;
@@ -272,14 +287,23 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
     const int64_t groups = n_rows * (npad / 8);
     const dim3 sgrid((unsigned)((groups + 255) / 256));
     const dim3 grid((unsigned)(row_tiles * col_tiles));
+    const WeighSpec wv = w ? *w : WeighSpec{};
     if (split) {
-        hipLaunchKernelGGL(split_bf16_kernel<true>, sgrid, dim3(256), 0, ctx->stream, in, n_rows, n,
-                           npad, phi, plo);
+        if (w)
+            hipLaunchKernelGGL((split_bf16_kernel<true, true>), sgrid, dim3(256), 0, ctx->stream, in,
+                               n_rows, n, npad, phi, plo, wv);
+        else
+            hipLaunchKernelGGL((split_bf16_kernel<true, false>), sgrid, dim3(256), 0, ctx->stream, in,
+                               n_rows, n, npad, phi, plo, wv);
         hipLaunchKernelGGL(ramp_gemm_kernel<true>, grid, dim3(256), 0, ctx->stream, phi, plo, out,
                            n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, scale);
     } else {
-        hipLaunchKernelGGL(split_bf16_kernel<false>, sgrid, dim3(256), 0, ctx->stream, in, n_rows, n,
-                           npad, phi, plo);
+        if (w)
+            hipLaunchKernelGGL((split_bf16_kernel<false, true>), sgrid, dim3(256), 0, ctx->stream, in,
+                               n_rows, n, npad, phi, plo, wv);
+        else
+            hipLaunchKernelGGL((split_bf16_kernel<false, false>), sgrid, dim3(256), 0, ctx->stream, in,
+                               n_rows, n, npad, phi, plo, wv);
         hipLaunchKernelGGL(ramp_gemm_kernel<false>, grid, dim3(256), 0, ctx->stream, phi, plo, out,
                            n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, scale);
     }

@@ -307,6 +307,26 @@ class Plan:
             call("mct_weight_views", self.h, _f32(sgm), int(view_begin), n, cp, len(keep))
         return sgm
 
+    def weight_filter(self, sgm: torch.Tensor, out: torch.Tensor, taps: np.ndarray, scale: float,
+                      method: str, view_begin: int = 0, bh_coef=None) -> torch.Tensor:
+        """weight_views(sgm, view_begin, bh_coef) then the context's
+        filter_rows(sgm, out, taps, scale, method) in one pass, sgm left
+        unweighted; MFMA methods only (bf16, bf16x3)."""
+        rows, n, nu = sgm.shape
+        if taps.shape[0] != 2 * nu - 1:
+            raise ValueError(f"expected {2 * nu - 1} taps, got {taps.shape[0]}")
+        if out.shape != sgm.shape:
+            raise ValueError("out must have the shape of sgm")
+        keep_t, tp = _host_f32(taps)
+        if bh_coef is None or len(bh_coef) == 0:
+            call("mct_weight_filter_rows", self.h, _f32(sgm), _f32(out), int(view_begin), n, None, 0,
+                 tp, float(scale), FILTER_METHODS[method])
+        else:
+            keep, cp = _host_f32(bh_coef)
+            call("mct_weight_filter_rows", self.h, _f32(sgm), _f32(out), int(view_begin), n, cp,
+                 len(keep), tp, float(scale), FILTER_METHODS[method])
+        return out
+
     def backproject_views(self, filt: torch.Tensor, dtype: str, layout: str, view_begin: int,
                           view_count: int, vol: torch.Tensor, finish: bool = False) -> torch.Tensor:
         """Adds the back-projection of views [view_begin, view_begin +

@@ -497,16 +497,24 @@ class Mgfbp:
 
     def weight_and_filter(self, sgm: torch.Tensor, view_begin: int | None = None) -> torch.Tensor:
         """BHCorrection + WeightSgm + FilterSinogram (run_mgfbp.py:63-70,
-        1282-1292) on a device FP32 sinogram (rows, views, n_u), weighted in
-        place; with view_begin, sgm is the block of views [view_begin,
-        view_begin + views) of the scan.  Returns the filtered sinogram."""
+        1282-1292) on a device FP32 sinogram (rows, views, n_u); with
+        view_begin, sgm is the block of views [view_begin, view_begin + views)
+        of the scan.  Returns the filtered sinogram.  With an MFMA ramp filter
+        and no vertical Gaussian the weighting runs inside the filter's input
+        pass (one read of the sinogram, the same bits) and sgm is left as
+        given; otherwise sgm is weighted in place."""
         ctx, plan = self.ctx(), self.plan()
         bh = self.array_bh_coefficients if self.bool_bh_correction else None
+        taps = self.recon_kernel()
+        if taps is not None and not self.apply_gauss_vertical and self.filter_method != "f32":
+            filt = torch.empty_like(sgm)
+            plan.weight_filter(sgm, filt, taps, self.det_elem_width, self.filter_method,
+                               0 if view_begin is None else view_begin, bh)
+            return filt
         if view_begin is None:
             plan.weight(sgm, bh)
         else:
             plan.weight_views(sgm, view_begin, bh)
-        taps = self.recon_kernel()
         if taps is None:
             # run_mgfbp.py:1283-1285: kernel 'None' filters nothing, the
             # vertical Gaussian included
@@ -522,7 +530,8 @@ class Mgfbp:
     def reconstruct(self, sgm: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """The device pipeline of MainFunction (run_mgfbp.py:63-81) on a
         resident FP32 sinogram (rows, views, n_u); sgm is weighted in place
-        (as the reference mutates its ndarray argument)."""
+        (as the reference mutates its ndarray argument) unless the weighting
+        runs inside the ramp filter's input pass (weight_and_filter)."""
         ctx, plan = self.ctx(), self.plan()
         filt = self.weight_and_filter(sgm)
         self.last_filtered = filt

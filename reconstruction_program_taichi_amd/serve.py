@@ -112,11 +112,15 @@ class PlanCache:
 
 # ------------------------------------------------------------------ device work
 def fbp_stages(ctx: Context, plan: Plan, job: Job, sgm: torch.Tensor):
-    """Weighting, ramp filter and repack of one FBP/FDK job on ctx's stream;
-    returns (packed taps, layout)."""
-    plan.weight(sgm)
+    """Weighting, ramp filter and repack of one FBP/FDK job on ctx's stream
+    (the MFMA filters weigh in their input pass); returns (packed taps,
+    layout)."""
     filt = torch.empty_like(sgm)
-    ctx.filter_rows(sgm, filt, job.taps, job.det_w, job.filter_method)
+    if job.filter_method == "f32":
+        plan.weight(sgm)
+        ctx.filter_rows(sgm, filt, job.taps, job.det_w, job.filter_method)
+    else:
+        plan.weight_filter(sgm, filt, job.taps, job.det_w, job.filter_method)
     layout = "quad" if job.spec.cone else "pair"
     return ctx.pack(filt, job.storage, layout), layout
 

@@ -91,6 +91,49 @@ def test_filter_rows_mfma(ctx, method, tol, n, rows):
     assert err <= tol
 
 
+@pytest.mark.parametrize("case", ["cone_bf16x3", "fan_short_bh_bf16", "fan_curved_block"])
+def test_weight_filter_fused_equals_separate(ctx, case):
+    """mct_weight_filter_rows = mct_weight_views then mct_filter_rows, bit for
+    bit (the same FP32 weighting, applied where the rows become BF16
+    planes), with the input sinogram left unweighted."""
+    from dataclasses import replace
+
+    bh, view_begin, count = None, 0, None
+    if case == "cone_bf16x3":
+        spec, method, w = cone_case(), "bf16x3", 0.6
+    elif case == "fan_short_bh_bf16":
+        spec, method, w = fan_case(scan_deg=220.0, rows=3), "bf16", 0.8
+        bh = [1.0, 0.05, -0.002]
+    else:  # views 17 .. 56 of a short scan on a curved detector
+        spec, method, w = replace(fan_case(scan_deg=250.0, rows=2), curved=True), "bf16x3", 0.8
+        view_begin, count = 17, 40
+    count = count or spec.n_views
+    rng = np.random.default_rng(21)
+    raw = rng.uniform(0, 2, (spec.n_v, count, spec.n_u)).astype(np.float32)
+    taps = recon.ramlak_taps(spec.n_u, w, np.float32)
+    plan = Plan(ctx, spec)
+    sep_in = dev(raw)
+    plan.weight_views(sep_in, view_begin, bh)
+    ref = torch.empty_like(sep_in)
+    ctx.filter_rows(sep_in, ref, taps, w, method)
+    sgm = dev(raw)
+    got = torch.empty_like(sgm)
+    plan.weight_filter(sgm, got, taps, w, method, view_begin, bh)
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    assert torch.equal(sgm.cpu(), torch.from_numpy(raw))
+    assert np.count_nonzero(host(got)) > got.numel() // 2
+
+
+def test_weight_filter_fused_rejects_the_f32_method(ctx):
+    spec = fan_case()
+    plan = Plan(ctx, spec)
+    sgm = torch.zeros((spec.n_v, spec.n_views, spec.n_u), device=DEV)
+    with pytest.raises(RuntimeError, match="MFMA"):
+        plan.weight_filter(sgm, torch.empty_like(sgm), recon.ramlak_taps(spec.n_u, 0.8, np.float32),
+                           0.8, "f32")
+
+
 @pytest.mark.parametrize("rows", [1, 7, 37, 48])
 def test_filter_vertical_matches_oracle(ctx, rows):
     from reconstruction_program_taichi_amd import geometry as geo
