@@ -75,11 +75,22 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict
     }
     if (WEIGH) {
         const int s = (int)(r / w.nviews), i = (int)(r - (int64_t)s * w.nviews);
+        float u[8];
+        if (m0 + 8 <= n) {  // the positions array is 256-byte aligned and m0 a multiple of 8
+            const float4 a = reinterpret_cast<const float4 *>(w.u + m0)[0];
+            const float4 b = reinterpret_cast<const float4 *>(w.u + m0)[1];
+            u[0] = a.x, u[1] = a.y, u[2] = a.z, u[3] = a.w;
+            u[4] = b.x, u[5] = b.y, u[6] = b.z, u[7] = b.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) u[q] = m0 + q < n ? w.u[m0 + q] : 0.f;
+        }
+        const float v = w.v[s];
 #pragma unroll
         for (int q = 0; q < 8; ++q)
             if (m0 + q < n)
-                x[q] = mct::weigh(x[q], m0 + q, s, w.view0 + i, w.u, w.v, w.ang, w.sdd, w.scan,
-                                  w.short_scan, w.curved, w.bh, w.bh_order);
+                x[q] = mct::weigh(x[q], u[q], v, w.view0 + i, w.ang, w.sdd, w.scan, w.short_scan,
+                                  w.curved, w.bh, w.bh_order);
     }
     bf16x8 h, l;
 #pragma unroll

@@ -13,11 +13,10 @@ struct BHCoef {
 };
 
 // BHCorrection (run_mgfbp.py:946-954) then WeightSgm (run_mgfbp.py:956-1011)
-// of one sample p at detector element j of row s in scan view i, operation
-// by operation in FP32; the library is built with -ffp-contract=off, so no
-// product is fused into an FMA here.
-__device__ __forceinline__ float weigh(float p, int j, int s, int i, const float *__restrict__ uarr,
-                                       const float *__restrict__ varr, const float *__restrict__ ang,
+// of one sample p at the detector element with positions (u, v) in scan view
+// i, operation by operation in FP32; the library is built with
+// -ffp-contract=off, so no product is fused into an FMA here.
+__device__ __forceinline__ float weigh(float p, float u, float v, int i, const float *__restrict__ ang,
                                        float sdd, float scan, int short_scan, int curved,
                                        const BHCoef &bh, int bh_order) {
     if (bh_order > 0) {
@@ -28,8 +27,6 @@ __device__ __forceinline__ float weigh(float p, int j, int s, int i, const float
         }
         p = acc;
     }
-    float u = uarr[j];
-    float v = varr[s];
     if (curved) {
         p = p * sdd * cosf((-1.f) * u / sdd) * sdd / sqrtf(sdd * sdd + v * v);
     } else {

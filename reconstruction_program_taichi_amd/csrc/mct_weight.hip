@@ -30,9 +30,10 @@ __global__ void weight_kernel(float *__restrict__ sgm, int nv, int nviews, int n
 #pragma unroll
         for (int q = 0; q < 4; ++q) p[q] = j0 + q < nu ? row[j0 + q] : 0.f;
     }
+    const float v = varr[s];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        p[q] = weigh(p[q], j0 + q < nu ? j0 + q : nu - 1, s, view0 + i, uarr, varr, ang, sdd, scan,
+        p[q] = weigh(p[q], uarr[j0 + q < nu ? j0 + q : nu - 1], v, view0 + i, ang, sdd, scan,
                      short_scan, curved, bh, bh_order);
     if (vec) {
         *reinterpret_cast<float4 *>(row + j0) = make_float4(p[0], p[1], p[2], p[3]);
