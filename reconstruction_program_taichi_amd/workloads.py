@@ -12,6 +12,11 @@
   filter: the ramp filter alone on the rows of the cone FDK sinogram
          (256 x 1080 rows of 1024 samples, Hamming t = 1 taps at 0.4 mm),
          seeded uniform input, with --filter's method.
+  helical: helical rebinning FBP (run_mgfbp_helical's path): 64 detector
+         rows x 736 columns at 1.0 mm, SID 570 / SDD 1040 mm, 1152 views per
+         turn over 3 turns, pitch 0.8, slices of 512 x 512 at 0.7 mm every
+         0.5 mm; seeded uniform sinogram; rebinning, then weighting, ramp
+         filter and the batched fan back-projection of every slice.
 Each returns a dict (printed as one JSON line by bench.py).
 """
 from __future__ import annotations
@@ -162,5 +167,42 @@ def filter_rows(args) -> dict:
     return line
 
 
+def helical(args) -> dict:
+    from .fbp_helical import Mgfbp_helical
+
+    turns, per_turn = 3, 1152
+    cfg = {"InputDir": "/nonexistent", "OutputDir": "/tmp/mct_workloads", "InputFiles": "x",
+           "OutputFilePrefix": "", "OutputFileReplace": ["a", "b"], "SinogramWidth": 736,
+           "DetectorElementWidth": 1.0, "DetectorOffcenter": 0.0, "SliceCount": 64,
+           "Views": per_turn * turns, "TotalScanAngle": 360.0 * turns,
+           "SourceIsocenterDistance": 570.0, "SourceDetectorDistance": 1040.0, "ImageDimension": 512,
+           "PixelSize": 0.7, "HammingFilter": 1.0, "ConeBeam": True, "DetectorElementHeight": 1.0,
+           "ImageSliceCount": 1, "VoxelHeight": 0.5, "SliceOffCenter": 0.0, "HelicalPitch": 0.8}
+    f = Mgfbp_helical(cfg)
+    rng = torch.Generator(device="cuda").manual_seed(5)
+    sgm = torch.rand((f.det_elem_count_vertical, f.view_num, f.det_elem_count_horizontal),
+                     generator=rng, device="cuda")
+    reb = f.rebin(sgm)
+    nz, vc, n = reb.shape
+    out = torch.empty((nz, 512, 512), dtype=torch.float32, device="cuda")
+    work = torch.empty_like(reb)
+
+    def step():
+        r = f.rebin(sgm)
+        f.reconstruct_rebinned(r, out)
+
+    def rebin_only():
+        work.copy_(f.rebin(sgm))
+
+    step()
+    t = _sync_time(step, max(args.steps, 3))
+    t_rebin = _sync_time(rebin_only, max(args.steps, 3))
+    U = float(nz) * 512 * 512 * vc
+    return {"workload": f"helical FBP: 64 rows x 736 columns, {per_turn} views/turn x {turns} turns, "
+                        f"pitch 0.8 -> {nz} slices of 512x512 from {vc} rebinned views each",
+            "ms_per_step": round(t * 1e3, 3), "rebin_ms": round(t_rebin * 1e3, 3),
+            "gups": round(U / t / 1e9, 2), "ramp_filter": f.filter_method}
+
+
 def run(name: str, args) -> dict:
-    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir, "filter": filter_rows}[name](args)
+    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir, "filter": filter_rows, "helical": helical}[name](args)
