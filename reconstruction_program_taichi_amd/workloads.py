@@ -12,6 +12,10 @@
   filter: the ramp filter alone on the rows of the cone FDK sinogram
          (256 x 1080 rows of 1024 samples, Hamming t = 1 taps at 0.4 mm),
          seeded uniform input, with --filter's method.
+  pmatrix: the cone back-projection of the FDK benchmark's geometry (512 x 512
+         x 256 at 0.3 mm, 1024 x 256 detector at 0.4 mm, SID 750 / SDD 1250 mm)
+         given as per-view P-matrices (the P-matrix branch, generic kernel,
+         FP32 QUAD taps) for the first --fp-views views; seeded uniform taps.
   helical: helical rebinning FBP (run_mgfbp_helical's path): 64 detector
          rows x 736 columns at 1.0 mm, SID 570 / SDD 1040 mm, 1152 views per
          turn over 3 turns, pitch 0.8, slices of 512 x 512 at 0.7 mm every
@@ -204,5 +208,56 @@ def helical(args) -> dict:
             "gups": round(U / t / 1e9, 2), "ramp_filter": f.filter_method}
 
 
+def circular_pmatrices(spec) -> np.ndarray:
+    """FP64 P-matrices (12 per view) of spec's circular flat-detector scan:
+    view j has its source at sid (cos b, sin b, 0), b = angle_j - img_rot,
+    detector columns along (-sin b, cos b, 0) from u[0] in steps of du, rows
+    along z from v[0] in steps of dv, at distance sdd; P = [A^-1 | -A^-1 S]
+    with A = [du e, dv e_z, sdd n + u[0] e + v[0] e_z] maps a point to
+    (u index, v index) after division by its third component."""
+    u, v = np.asarray(spec.u, np.float64), np.asarray(spec.v, np.float64)
+    du, dv = u[1] - u[0], v[1] - v[0]
+    out = np.empty((spec.n_views, 12))
+    for j, b in enumerate(np.asarray(spec.angles, np.float64) - spec.img_rot):
+        c, s = np.cos(b), np.sin(b)
+        e, n, ez = np.array([-s, c, 0.0]), -np.array([c, s, 0.0]), np.array([0.0, 0.0, 1.0])
+        A = np.column_stack([du * e, dv * ez, spec.sdd * n + u[0] * e + v[0] * ez])
+        Ai = np.linalg.inv(A)
+        out[j] = np.hstack([Ai, -Ai @ (spec.sid * np.array([c, s, 0.0]))[:, None]]).reshape(12)
+    return out.reshape(-1)
+
+
+def pmatrix(args) -> dict:
+    from .fbp import Mgfbp
+
+    nv = int(args.fp_views)
+    cfg = {"InputDir": "/nonexistent", "OutputDir": "/tmp/mct_workloads", "InputFiles": "x",
+           "OutputFilePrefix": "", "OutputFileReplace": ["a", "b"],
+           "DetectorElementCountHorizontal": 1024, "DetectorElementWidth": 0.4,
+           "DetectorOffsetHorizontal": 0.0, "DetectorElementCountVertical": 256, "Views": 1080,
+           "TotalScanAngle": 360.0, "SourceIsocenterDistance": 750.0, "SourceDetectorDistance": 1250.0,
+           "ImageDimension": 512, "PixelSize": 0.3, "HammingFilter": 1.0, "ConeBeam": True,
+           "DetectorElementHeight": 0.4, "ImageDimensionZ": 256, "VoxelHeight": 0.3,
+           "DetectorOffsetVertical": 0.0, "ImageCenterZ": 0.0}
+    fbp = Mgfbp(cfg)
+    spec = fbp.geometry_spec()
+    if nv < spec.n_views:  # leading subset of the views (same per-view work)
+        spec = replace(spec, n_views=nv, angles=spec.angles[:nv])
+    spec = replace(spec, pmatrix=circular_pmatrices(spec).astype(np.float32))
+    ctx = fbp.ctx()
+    plan = Plan(ctx, spec)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    filt = torch.rand((spec.n_v, spec.n_views, spec.n_u), generator=g, device="cuda")
+    packed = ctx.pack(filt, "f32", "quad")
+    vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
+    plan.backproject_fbp(packed, "f32", "quad", vol)
+    kernel = ctx.bp_last_kernel()
+    t = _sync_time(lambda: plan.backproject_fbp(packed, "f32", "quad", vol), max(1, min(args.steps, 3)))
+    U = 512.0 * 512 * 256 * spec.n_views
+    return {"workload": f"P-matrix cone back-projection 512x512x256 from {spec.n_views} views x 256 x 1024",
+            "kernel": kernel, "ms": round(t * 1e3, 3), "gups": round(U / t / 1e9, 2)}
+
+
 def run(name: str, args) -> dict:
-    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir, "filter": filter_rows, "helical": helical}[name](args)
+    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir, "filter": filter_rows, "helical": helical,
+            "pmatrix": pmatrix}[name](args)
