@@ -55,6 +55,15 @@ def imreadRaw(path: str, height: int, width: int, dtype=np.float32, nSlice: int 
         return arr
 
 
+def imaddRaw(img, path: str, dtype=None, idx=1) -> None:
+    """run_mgfpj.py:708-722: write the raw bytes of img to path when idx is 0,
+    append them otherwise (converted to dtype first when given)."""
+    if dtype is not None:
+        img = img.astype(dtype)
+    with open(path, "wb" if idx == 0 else "ab") as fp:
+        fp.write(np.ascontiguousarray(img).tobytes())
+
+
 def imwriteRaw(img: np.ndarray, path: str, dtype=np.float32) -> None:
     """Raw C-order bytes (the reference uses crip.io.imwriteRaw)."""
     np.ascontiguousarray(img, dtype=dtype).tofile(path)
