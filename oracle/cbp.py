@@ -21,6 +21,16 @@ def lib():
         _lib.bp_oracle_fbp.argtypes = [fp, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp, C.c_int,
                                        C.c_int, C.c_int, C.c_int] + [C.c_float] * 9 + [C.c_int, fp]
         _lib.bp_oracle_fbp.restype = C.c_int
+        for T, ct in (("f32", C.c_float), ("f64", C.c_double)):
+            pt = C.POINTER(ct)
+            fn = getattr(_lib, f"fp_oracle_{T}")
+            fn.argtypes = ([pt, C.c_int, C.c_int] + [ct] * 5 + [pt, ct, ct, pt, C.c_int, pt, C.c_int, pt,
+                           C.c_int, C.c_int, C.c_int, ct, C.c_int, ct, pt])
+            fn.restype = C.c_int
+            fn = getattr(_lib, f"bpir_oracle_{T}")
+            fn.argtypes = ([pt, C.c_int, C.c_int, C.c_int, pt, pt, ct, ct, pt, pt, C.c_int, C.c_int]
+                           + [ct] * 5 + [pt, ct, ct, ct, C.c_int, pt])
+            fn.restype = C.c_int
     return _lib
 
 
@@ -45,4 +55,73 @@ def backproject_fbp(filt, g, z0=0, z1=None):
                         g["pix"], g.get("vox_h", 0.0), g.get("cx", 0.0), g.get("cy", 0.0),
                         g.get("cz", 0.0), g["sid"], g["sdd"], g.get("det_h", 1.0), div,
                         int(bool(g.get("cone", False))), _p(out))
+    return out
+
+
+def _arr(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _pt(a):
+    ct = C.c_float if a.dtype == np.float32 else C.c_double
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def forward_project(vol, g, oversample=1, ray_mask=None, count_mode=0, dtype=np.float32):
+    """recon.forward_project (run_mgfpj.py:463-616) through fp_oracle.c:
+    flat or curved detector, cone or fan, no P-matrix; same arithmetic and
+    expression order.  vol (nz, ny, nx) -> (rows, views, n_u)."""
+    if g.get("matrix_a") is not None:
+        raise ValueError("the C forward projector has no P-matrix branch")
+    T = np.dtype(dtype).type
+    vol = _arr(vol, dtype)
+    nz, _, nx = vol.shape
+    ang, u, v = _arr(g["angles"], dtype), _arr(g["u"], dtype), _arr(g["v"], dtype)
+    nrows, nviews, nuo = v.shape[0], ang.shape[0], u.shape[0]
+    czs = g.get("center_z")
+    czs = None if czs is None else _arr(czs, dtype)
+    out = np.empty((nrows, nviews, nuo), dtype=dtype)
+    fn = getattr(lib(), "fp_oracle_f32" if dtype == np.float32 else "fp_oracle_f64")
+    fn(_pt(vol), nx, nz, T(g["pix"]), T(g.get("vox_h", 0.0)), T(g.get("cx", 0.0)), T(g.get("cy", 0.0)),
+       T(g.get("cz", 0.0)), None if czs is None else _pt(czs), T(g["sid"]), T(g["sdd"]), _pt(ang),
+       nviews, _pt(u), nuo, _pt(v), nrows, int(bool(g.get("cone", False))),
+       int(bool(g.get("curved", False))), T(g.get("fpj_step", 0.2)), int(count_mode),
+       T(g.get("zpv", 0.0)), _pt(out))
+    if oversample > 1:
+        b = out.reshape(nrows, nviews, nuo // oversample, oversample)
+        s = np.zeros(b.shape[:3], dtype=dtype)
+        for i in range(oversample):
+            s = s + b[..., i]
+        out = s / T(oversample)
+    if ray_mask is not None:
+        out = np.where(np.asarray(ray_mask) == 1, out, T(0))
+    return out.astype(dtype)
+
+
+def backproject_ir(sgm, g, ray_mask=None, dtype=np.float32):
+    """recon.backproject_ir (run_mgfbp_iterative_recon_IRN_CGLS.py:372-481)
+    through fp_oracle.c: axial view, flat detector, no P-matrix."""
+    from .recon import view_table
+
+    if g.get("pmatrix") is not None or g.get("curved", False):
+        raise ValueError("the C IR back-projector covers the flat, non-P-matrix geometry")
+    T = np.dtype(dtype).type
+    f = _arr(sgm, dtype)
+    if ray_mask is not None:
+        f = _arr(f * np.asarray(ray_mask, dtype=dtype), dtype)
+    nrows, nviews, nu = f.shape
+    rot = g.get("img_rot", 0.0)
+    c, s, _ = view_table(np.asarray(g["angles"], dtype=dtype), rot, dtype)
+    c, s = _arr(c, dtype), _arr(s, dtype)
+    cr, sr = T(np.cos(np.float64(T(rot)))), T(np.sin(np.float64(T(rot))))
+    u, v = _arr(g["u"], dtype), _arr(g["v"], dtype)
+    nx, nz = g["img_dim"], g["img_dim_z"]
+    czs = g.get("center_z")
+    czs = None if czs is None else _arr(czs, dtype)
+    out = np.empty((nz, nx, nx), dtype=dtype)
+    fn = getattr(lib(), "bpir_oracle_f32" if dtype == np.float32 else "bpir_oracle_f64")
+    fn(_pt(f), nrows, nviews, nu, _pt(c), _pt(s), cr, sr, _pt(u), _pt(v), nx, nz, T(g["pix"]),
+       T(g.get("vox_h", 0.0)), T(g.get("cx", 0.0)), T(g.get("cy", 0.0)), T(g.get("cz", 0.0)),
+       None if czs is None else _pt(czs), T(g["sid"]), T(g["sdd"]), T(g.get("det_h", 1.0)),
+       int(bool(g.get("cone", False))), _pt(out))
     return out

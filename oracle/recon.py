@@ -728,12 +728,15 @@ def cg_solve(apply_A, b, x0, iters):
 
 
 def ir_solve(sgm, g, x0, lam, beta_tv, n_irn, n_iter, couple_z=True, prior=None, alpha=0.0,
-             dtype=np.float64):
+             dtype=np.float64, projectors=None):
     """Mgfbp_ir.MainFunction / TikhonovSol (run_mgfbp_iterative_recon_IRN_CGLS.py:
     157-232) and, with a prior, Mgfbp_ir_piccs (…_piccs.py:83-150), with the
     oracle's projectors: ray map, P^T b, then 1 + n_irn reweighted CG passes of
     n_iter iterations.  couple_z=False drops the z difference (independent
-    slices).  Returns x."""
+    slices).  projectors: (forward_project, backproject_ir) with this module's
+    signatures (e.g. the C restatements of oracle/cbp.py, for large cases);
+    None uses this module's.  Returns x."""
+    fwd, bwd = projectors if projectors is not None else (forward_project, backproject_ir)
     sgm = np.asarray(sgm, dtype=dtype)
     rmask = ray_mask(g, dtype=dtype)
     nz = x0.shape[0]
@@ -750,11 +753,11 @@ def ir_solve(sgm, g, x0, lam, beta_tv, n_irn, n_iter, couple_z=True, prior=None,
         return np.stack([dtwd(x[k:k + 1], w[k:k + 1])[0] for k in range(nz)])
 
     def fpbp(x):
-        fp = forward_project(x, g, ray_mask=rmask, dtype=dtype)
-        return backproject_ir(fp, g, dtype=dtype).astype(np.float64)
+        fp = fwd(x, g, ray_mask=rmask, dtype=dtype)
+        return bwd(fp, g, dtype=dtype).astype(np.float64)
 
     ratio = sgm.size / x0.size
-    bpb = backproject_ir(sgm, g, ray_mask=rmask, dtype=dtype).astype(np.float64)
+    bpb = bwd(sgm, g, ray_mask=rmask, dtype=dtype).astype(np.float64)
     x = np.asarray(x0 if prior is None else prior, dtype=np.float64)
     c1 = lam * (1 - alpha) * ratio
     c2 = lam * alpha * ratio

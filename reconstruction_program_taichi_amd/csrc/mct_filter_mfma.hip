@@ -1,23 +1,34 @@
 // Ramp filter as a Toeplitz GEMM on MFMA (Mgfbp.ConvolveSgmAndKernel,
-// run_mgfbp.py:1033-1046):  OUT[r, k] = scale * sum_m IN[r, m] T[m, k],
-// T[m, k] = taps[k + n-1 - m].
+// run_mgfbp.py:1033-1046):  OUT[r, k] = scale * sum_m IN[r, m] taps[k + n-1 - m].
+//
+// Difference form.  The ramp taps sum to almost zero, so for a smooth row the
+// products x_m h_{k-m} cancel to an output far smaller than themselves, and
+// any rounding of the operands is amplified by that ratio (weighted
+// projections of ~2e3 filter to ~1).  Summation by parts removes it: with
+// g_j = h_{-(n-1)} + ... + h_j the running sum of the taps (g_{-n} = 0) and
+// D_m = x_m - x_{m-1} the first difference of the row (x_{-1} = x_n = 0),
+//     sum_{m<n} x_m h_{k-m} = sum_{m<=n} D_m g_{k-m}
+// exactly.  The MFMA operands are then D (small where the row is smooth) and
+// g (bounded by about h_0 / 2), so the BF16 rounding scales with |D| instead
+// of |x|.  The product is still Toeplitz: OUT[r, k] = scale * sum_{m<=n}
+// D[r, m] G[k + n - m] with G[i] = taps[0] + ... + taps[i-1] (2n entries,
+// G[0] = 0), over K = n + 1 samples.
 //
 // Two launches:
-//   * split_bf16_kernel converts every input row once into BF16 planes
-//     [rows][npad]: hi = bf16(x) and, for the split method, lo = bf16(x - hi),
-//     zero past n; with a WeighSpec it first weighs each sample exactly as
-//     the weighting kernel does (same FP32 operations), so weighting and
-//     filtering read the sinogram once and the planes are those of the
-//     weighted sinogram;
+//   * split_bf16_kernel turns every input row once into BF16 planes of its
+//     first differences, [rows][npad] with npad >= n + 1: hi = bf16(D) and,
+//     for the split method, lo = bf16(D - hi), zero past n; with a WeighSpec
+//     it first weighs each sample exactly as the weighting kernel does (same
+//     FP32 operations), so weighting and filtering read the sinogram once;
 //   * ramp_gemm_kernel: workgroup tile 128 rows x 128 output columns, four
 //     wavefronts of 64 x 64, each a 4 x 4 grid of v_mfma_f32_16x16x32_bf16
 //     accumulators (FP32).  Per K step of 32 the A tile (128 rows x 32
 //     samples of each plane) is copied from the planes into one of two LDS
 //     buffers; the copy for step s+1 is in flight while step s's MFMAs run.
 //     B is never materialised: every B fragment is 8 consecutive entries of
-//     the reversed tap window W[i] = taps[T - i], T = k0 + 127 + n-1, kept in
-//     LDS as 8 copies shifted by 0..7 so each fragment is one aligned 16-byte
-//     LDS read; a window covers KC samples of K and is restaged every KC/32
+//     the reversed window W[i] = G[T - i], T = k0 + 127 + n, kept in LDS as 8
+//     copies shifted by 0..7 so each fragment is one aligned 16-byte LDS
+//     read; a window covers KC samples of K and is restaged every KC/32
 //     steps.  Workgroup ids are remapped so the 8 column tiles of a row tile
 //     run on one XCD and share its L2.
 // BF16: hi x hi.  BF16X3: hi x hi + hi x lo + lo x hi (relative product error
@@ -51,7 +62,8 @@ __device__ __forceinline__ __bf16 to_bf16(float f) {
     return __builtin_bit_cast(__bf16, mct::f32_to_bf16_rn(f));
 }
 
-// One thread per 8 samples of a row: in[r][m0 .. m0+7] -> hi/lo planes
+// One thread per 8 samples of a row: the differences D[m0 .. m0+7] of
+// in[r][m0-1 .. m0+7] (samples outside [0, n) are 0) -> hi/lo planes
 // (WEIGH: row r is (detector row r / w.nviews, view r % w.nviews of the block)).
 template <bool SPLIT, bool WEIGH>
 __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict__ in, int64_t rows,
@@ -64,6 +76,7 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict
     const int m0 = (int)(t - r * groups) * 8;
     const float *src = in + r * n + m0;
     float x[8];
+    float xp = (m0 >= 1 && m0 - 1 < n) ? src[-1] : 0.f;  // sample m0 - 1
     if (m0 + 8 <= n && ((((uintptr_t)src) & 15) == 0)) {
         const float4 a = reinterpret_cast<const float4 *>(src)[0];
         const float4 b = reinterpret_cast<const float4 *>(src)[1];
@@ -91,12 +104,16 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict
             if (m0 + q < n)
                 x[q] = mct::weigh(x[q], u[q], v, w.view0 + i, w.ang, w.sdd, w.scan, w.short_scan,
                                   w.curved, w.bh, w.bh_order);
+        if (m0 >= 1 && m0 - 1 < n)
+            xp = mct::weigh(xp, w.u[m0 - 1], v, w.view0 + i, w.ang, w.sdd, w.scan, w.short_scan,
+                            w.curved, w.bh, w.bh_order);
     }
     bf16x8 h, l;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        h[q] = to_bf16(x[q]);
-        if (SPLIT) l[q] = to_bf16(x[q] - (float)h[q]);
+        const float d = x[q] - (q ? x[q - 1] : xp);
+        h[q] = to_bf16(d);
+        if (SPLIT) l[q] = to_bf16(d - (float)h[q]);
     }
     *reinterpret_cast<bf16x8 *>(hi + r * npad + m0) = h;
     if (SPLIT) *reinterpret_cast<bf16x8 *>(lo + r * npad + m0) = l;
@@ -119,8 +136,8 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(const __bf16 *__restrict
     const int tile = mct::xcd_remap((int)blockIdx.x, (int)gridDim.x, nxcd);
     const int64_t r0 = (int64_t)(tile / col_tiles) * TM;
     const int k0 = (tile % col_tiles) * TN;
-    const int ntaps = 2 * n - 1;
-    const int T = k0 + (TN - 1) + n - 1;
+    const int ntaps = 2 * n;  // G[0 .. 2n-1]
+    const int T = k0 + (TN - 1) + n;
 
     f32x4 acc[4][4];
 #pragma unroll
@@ -246,15 +263,19 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
                      const float *taps, float scale, int method, const WeighSpec *w) {
     MCT_CHECK(method == MCT_FILTER_BF16_MFMA || method == MCT_FILTER_BF16X3_MFMA,
               "unknown filter method %d", method);
-    const int ntaps = 2 * n - 1;
-    // split taps into BF16 hi + lo once per tap set (cached in the context)
-    const uint64_t h = fnv1a(taps, sizeof(float) * (size_t)ntaps, (uint64_t)n);
+    const int ntaps = 2 * n;  // running sums G[0 .. 2n-1] of the 2n-1 taps
+    // running sums of the taps (FP64), split into BF16 hi + lo once per tap
+    // set (cached in the context)
+    const uint64_t h = fnv1a(taps, sizeof(float) * (size_t)(2 * n - 1), (uint64_t)n);
     if (!ctx->toep || ctx->toep_n != n || ctx->toep_hash != h) {
         std::vector<uint16_t> hl(2 * (size_t)ntaps);
+        double g = 0.0;
         for (int i = 0; i < ntaps; ++i) {
-            uint16_t hi = bf16_bits_rn(taps[i]);
+            if (i) g += (double)taps[i - 1];
+            const float gf = (float)g;
+            uint16_t hi = bf16_bits_rn(gf);
             hl[i] = hi;
-            hl[ntaps + i] = bf16_bits_rn(taps[i] - bf16_value(hi));
+            hl[ntaps + i] = bf16_bits_rn(gf - bf16_value(hi));
         }
         const int64_t bytes = (int64_t)hl.size() * 2;
         if (ctx->toep_bytes < bytes) {
@@ -275,7 +296,7 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
     const __bf16 *thi = reinterpret_cast<const __bf16 *>(ctx->toep);
     const __bf16 *tlo = thi + ntaps;
     const bool split = method == MCT_FILTER_BF16X3_MFMA;
-    const int npad = (n + TK - 1) / TK * TK;
+    const int npad = (n + 1 + TK - 1) / TK * TK;  // K = n + 1 differences D_0 .. D_n
     const int64_t row_tiles = (n_rows + TM - 1) / TM;
     const int col_tiles = (n + TN - 1) / TN;
     MCT_CHECK(row_tiles * col_tiles <= INT32_MAX, "too many rows for one launch (%lld)",

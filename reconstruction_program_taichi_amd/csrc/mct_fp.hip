@@ -686,9 +686,13 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         // wavefronts (toward 256 wavefronts per CU in all, so the last round
         // of wavefronts is a small part of the launch) and add the partial
         // sums in a second pass.  (512^2 fan slice, 240 views: 0.46 ms with
-        // 32 wavefronts per CU, 0.35 ms with 256.)
+        // 32 wavefronts per CU, 0.35 ms with 256.)  The split counts the
+        // wavefronts that carry rays (rows not padded to bands of 4), so a
+        // problem of n rows with the target scaled by n splits each ray
+        // exactly as a one-row problem does.
         const int64_t target = (int64_t)ctx->cu_count * (ctx->fp_split ? ctx->fp_split : 256);
-        int64_t nseg = (target + nwaves - 1) / nwaves;
+        const int64_t nwaves_rays = nub * a.nviews * a.nrows;
+        int64_t nseg = (target + nwaves_rays - 1) / nwaves_rays;
         nseg = nseg < 1 ? 1 : (nseg > 16 ? 16 : nseg);
         const int64_t nblocks = (nwaves * nseg + 3) / 4;
         // 32-bit record offsets; gridDim.z of the pack launch; wavefront index

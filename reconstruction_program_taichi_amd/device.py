@@ -37,6 +37,17 @@ def _f32(t: torch.Tensor) -> int:
     return _ptr(t)
 
 
+def _shape(t: torch.Tensor, want: tuple, name: str) -> None:
+    """The kernels size every launch from the plan's geometry, so a buffer of
+    another shape would be read or written out of bounds: refuse it."""
+    if tuple(t.shape) != tuple(want):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, the plan expects {tuple(want)}")
+
+
+def _packed_bytes(rows: int, views: int, nu: int, dtype: str, layout: str) -> int:
+    return int(_abi.lib().mct_packed_bytes(rows, views, nu, DTYPES[dtype], LAYOUTS[layout]))
+
+
 def _host_f32(a) -> tuple[np.ndarray, C.POINTER(C.c_float)]:
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a, a.ctypes.data_as(C.POINTER(C.c_float))
@@ -138,6 +149,10 @@ class Context:
     def filter_vertical(self, inp: torch.Tensor, out: torch.Tensor, kern: np.ndarray,
                         det_h: float) -> torch.Tensor:
         rows, views, nu = inp.shape
+        if out.shape != inp.shape:
+            raise ValueError("out must have the shape of inp")
+        if np.asarray(kern).shape[0] != 2 * rows - 1:
+            raise ValueError(f"expected {2 * rows - 1} vertical taps, got {np.asarray(kern).shape[0]}")
         keep, kp = _host_f32(kern)
         call("mct_filter_vertical", self.h, _f32(inp), _f32(out), rows, views, nu, kp, float(det_h))
         return out
@@ -165,6 +180,8 @@ class Context:
         for i, (plan, filt, dtype, layout, vol) in enumerate(jobs):
             if plan.ctx is not self:
                 raise ValueError("plan belongs to another context")
+            plan._check_packed(filt, dtype, layout, plan.spec.n_views)
+            plan._check_vol(vol)
             arr[i] = _abi.BpJob(plan.h.value, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
         return arr
 
@@ -283,7 +300,32 @@ class Plan:
         except Exception:
             pass
 
+    # ---- shape checks against the plan's geometry (before any device call) --------
+    def _check_sgm(self, t: torch.Tensor, name: str = "sinogram", views: int | None = None,
+                   nu: int | None = None) -> None:
+        s = self.spec
+        _shape(t, (s.n_v, s.n_views if views is None else views, s.n_u if nu is None else nu), name)
+
+    def _check_vol(self, t: torch.Tensor) -> None:
+        s = self.spec
+        _shape(t, (s.img_dim_z, s.img_dim, s.img_dim), "volume")
+
+    def _check_packed(self, t: torch.Tensor, dtype: str, layout: str, views: int) -> None:
+        s = self.spec
+        need = _packed_bytes(s.n_v, views, s.n_u, dtype, layout)
+        if need <= 0:
+            raise ValueError(f"no {layout} packing for ({s.n_v}, {views}, {s.n_u})")
+        have = t.numel() * t.element_size()
+        if have < need or not t.is_contiguous():
+            raise ValueError(f"packed sinogram holds {have} bytes, the plan needs {need} contiguous bytes")
+
+    def _check_views(self, view_begin: int, view_count: int) -> None:
+        if view_begin < 0 or view_count < 0 or view_begin + view_count > self.spec.n_views:
+            raise ValueError(f"views [{view_begin}, {view_begin + view_count}) outside the plan's "
+                             f"{self.spec.n_views} views")
+
     def weight(self, sgm: torch.Tensor, bh_coef=None) -> torch.Tensor:
+        self._check_sgm(sgm)
         if bh_coef is None or len(bh_coef) == 0:
             call("mct_weight_sinogram", self.h, _f32(sgm), None, 0)
         else:
@@ -293,6 +335,8 @@ class Plan:
 
     def backproject_fbp(self, filt: torch.Tensor, dtype: str, layout: str,
                         vol: torch.Tensor) -> torch.Tensor:
+        self._check_packed(filt, dtype, layout, self.spec.n_views)
+        self._check_vol(vol)
         call("mct_backproject_fbp", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
         return vol
 
@@ -300,6 +344,8 @@ class Plan:
         """weight() of a block (rows, view_count, n_u) holding views
         [view_begin, view_begin + view_count) of the scan."""
         n = int(sgm.shape[1])
+        self._check_sgm(sgm, views=n)
+        self._check_views(view_begin, n)
         if bh_coef is None or len(bh_coef) == 0:
             call("mct_weight_views", self.h, _f32(sgm), int(view_begin), n, None, 0)
         else:
@@ -313,10 +359,11 @@ class Plan:
         filter_rows(sgm, out, taps, scale, method) in one pass, sgm left
         unweighted; MFMA methods only (bf16, bf16x3)."""
         rows, n, nu = sgm.shape
+        self._check_sgm(sgm, views=n)
+        self._check_sgm(out, "out", views=n)
+        self._check_views(view_begin, n)
         if taps.shape[0] != 2 * nu - 1:
             raise ValueError(f"expected {2 * nu - 1} taps, got {taps.shape[0]}")
-        if out.shape != sgm.shape:
-            raise ValueError("out must have the shape of sgm")
         keep_t, tp = _host_f32(taps)
         if bh_coef is None or len(bh_coef) == 0:
             call("mct_weight_filter_rows", self.h, _f32(sgm), _f32(out), int(view_begin), n, None, 0,
@@ -332,19 +379,36 @@ class Plan:
         """Adds the back-projection of views [view_begin, view_begin +
         view_count) (filt: their packed block) to vol; finish on the last
         block clears the truncation marks."""
+        self._check_views(view_begin, view_count)
+        self._check_packed(filt, dtype, layout, view_count)
+        self._check_vol(vol)
         call("mct_backproject_views", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout],
              int(view_begin), int(view_count), int(bool(finish)), _f32(vol))
         return vol
 
     def forward_project(self, vol: torch.Tensor, out: torch.Tensor, ray_mask=None,
                         oversample: int = 1) -> torch.Tensor:
-        call("mct_forward_project", self.h, _f32(vol), _f32(out), _ptr(ray_mask), int(oversample))
+        s = self.spec
+        if oversample < 1 or s.n_u % oversample:
+            raise ValueError(f"n_u ({s.n_u}) is not a multiple of oversample ({oversample})")
+        self._check_vol(vol)
+        self._check_sgm(out, "out", nu=s.n_u // oversample)
+        if ray_mask is not None:
+            self._check_sgm(ray_mask, "ray_mask", nu=s.n_u // oversample)
+        call("mct_forward_project", self.h, _f32(vol), _f32(out),
+             None if ray_mask is None else _f32(ray_mask), int(oversample))
         return out
 
     def ray_mask(self, out: torch.Tensor) -> torch.Tensor:
+        self._check_sgm(out, "ray_mask")
         call("mct_ray_mask", self.h, _f32(out))
         return out
 
     def backproject_ir(self, sgm: torch.Tensor, ray_mask, vol: torch.Tensor) -> torch.Tensor:
-        call("mct_backproject_ir", self.h, _f32(sgm), _ptr(ray_mask), _f32(vol))
+        self._check_sgm(sgm)
+        if ray_mask is not None:
+            self._check_sgm(ray_mask, "ray_mask")
+        self._check_vol(vol)
+        call("mct_backproject_ir", self.h, _f32(sgm), None if ray_mask is None else _f32(ray_mask),
+             _f32(vol))
         return vol

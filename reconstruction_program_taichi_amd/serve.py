@@ -31,7 +31,7 @@ from __future__ import annotations
 import math
 import time
 from collections import OrderedDict, deque
-from dataclasses import dataclass, field, replace
+from dataclasses import dataclass, field, fields, replace
 
 import numpy as np
 import torch
@@ -83,9 +83,18 @@ class Job:
 
 
 def geometry_key(spec: geo.GeometrySpec) -> tuple:
-    return (spec.img_dim, spec.img_dim_z, spec.pix, spec.vox_h, spec.n_views, spec.sid, spec.sdd,
-            spec.total_scan_angle, spec.cone, spec.det_h, spec.fpj_step, spec.n_u, spec.n_v,
-            spec.angles.tobytes(), spec.u.tobytes(), spec.v.tobytes())
+    """Every field of the geometry (arrays by dtype, shape and bytes): two
+    jobs share a plan, or a stacked CGLS run, only when their geometry is
+    the same in every respect the kernels read."""
+    key = []
+    for f in fields(spec):
+        if f.name.startswith("_"):
+            continue
+        v = getattr(spec, f.name)
+        if isinstance(v, np.ndarray):
+            v = (v.dtype.str, v.shape, v.tobytes())
+        key.append((f.name, v))
+    return tuple(key)
 
 
 class PlanCache:

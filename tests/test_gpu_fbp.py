@@ -78,7 +78,8 @@ def test_filter_rows_f32(ctx, n, rows):
 @pytest.mark.parametrize("n,rows", [(1024, 300), (800, 129), (100, 7), (37, 1)])
 def test_filter_rows_mfma(ctx, method, tol, n, rows):
     """Toeplitz GEMM on MFMA against the FP64 oracle: BF16 operands (relative
-    rounding 2^-9 per operand) and split-BF16 (about 2^-17)."""
+    rounding 2^-9 per operand: the rows' first differences and the taps'
+    running sums) and split-BF16 (about 2^-17)."""
     rng = np.random.default_rng(n + rows)
     x = rng.standard_normal((rows, n)).astype(np.float32)
     taps = recon.ramlak_taps(n, 0.4, np.float32)

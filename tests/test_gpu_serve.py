@@ -124,11 +124,13 @@ def _cgls_jobs(n, seed=21):
     return jobs
 
 
-def test_stacked_cgls_equals_single_runs():
-    """Three CGLS jobs run as one stacked problem give each job's own
-    iterates bit for bit (slices independent, per-job scalars, the same ray
-    split as one slice)."""
-    jobs = _cgls_jobs(3)
+@pytest.mark.parametrize("n", [3, 5])
+def test_stacked_cgls_equals_single_runs(n):
+    """n CGLS jobs run as one stacked problem give each job's own iterates
+    bit for bit (slices independent, per-job scalars, the same ray split as
+    one slice: with 5 slices the forward projector's row bands are padded,
+    and the split still follows the rays)."""
+    jobs = _cgls_jobs(n)
     ctx = Context(0)
     dev = torch.device("cuda", 0)
     single = []
@@ -138,12 +140,12 @@ def test_stacked_cgls_equals_single_runs():
         serve.cgls_run(ctx, plan, j, j.sgm_host.to(dev), x)
         single.append(x.clone())
         plan.close()
-    plan = Plan(ctx, serve.cgls_stack_spec(jobs[0].spec, 3))
+    plan = Plan(ctx, serve.cgls_stack_spec(jobs[0].spec, n))
     sgm = torch.cat([j.sgm_host.to(dev) for j in jobs])
-    x = torch.empty((3,) + tuple(jobs[0].vol_shape()[1:]), dtype=torch.float32, device=dev)
+    x = torch.empty((n,) + tuple(jobs[0].vol_shape()[1:]), dtype=torch.float32, device=dev)
     serve.cgls_run_stack(ctx, plan, jobs, sgm, x)
     torch.cuda.synchronize()
-    for i in range(3):
+    for i in range(n):
         assert torch.equal(x[i:i + 1], single[i]), f"job {i} differs"
     assert not torch.equal(single[0], single[1])
     plan.close()
@@ -165,3 +167,36 @@ def test_server_stacks_cgls_jobs():
     assert st["jobs"] == 6 and sum(stacks) == 6 and max(stacks) > 1, stacks
     for a, b in zip(jobs, plain):
         assert torch.equal(a.result, b.result), f"job {a.jid} differs"
+
+
+def test_jobs_differing_in_one_geometry_field_do_not_share_plans():
+    """Two FDK jobs that differ only in the image centre along z, and two FBP
+    jobs that differ only in the short-scan flag: the server keys plans on
+    the whole geometry, so each result equals its own plain run (and the two
+    of a pair differ)."""
+    from dataclasses import replace
+
+    mix = serve.make_mix(24, qps=400.0, seed=13, scale=0.12, kinds=("fdk256", "fbp256"),
+                         prototypes_per_kind=1)
+    from reconstruction_program_taichi_amd import geometry as geo
+
+    fdk = next(j for j in mix if j.kind == "fdk")
+    fbp = next(j for j in mix if j.kind == "fbp")
+    # a 450-degree scan: the short-scan flag switches the redundancy ramp on
+    scan = 2.5 * geo.PI
+    fan = replace(fbp.spec, total_scan_angle=scan, angles=geo.angle_array(fbp.spec.n_views, scan, 0.0))
+    jobs = [replace(fdk.fresh(), jid=0, arrival=0.0),
+            replace(fdk.fresh(), jid=1, arrival=0.0, spec=replace(fdk.spec, cz=0.75 * fdk.spec.vox_h)),
+            replace(fbp.fresh(), jid=2, arrival=0.0, spec=fan),
+            replace(fbp.fresh(), jid=3, arrival=0.0, spec=replace(fan, short_scan=True))]
+    plain = [j.fresh() for j in jobs]
+    srv = serve.Server(hbm_budget=8 << 30)
+    try:
+        srv.run(jobs)
+    finally:
+        srv.close()
+    serve.run_plain(plain)
+    for a, b in zip(jobs, plain):
+        assert torch.equal(a.result, b.result), f"job {a.jid} differs from its plain run"
+    assert not torch.equal(jobs[0].result, jobs[1].result)
+    assert not torch.equal(jobs[2].result, jobs[3].result)

@@ -155,3 +155,70 @@ def test_div_factor_cases():
     assert geo.div_factor(geo.PI + 0.3, True) == 1.0
     assert geo.div_factor(2 * geo.PI + 1.0, True) == pytest.approx(0.5)
     assert geo.div_factor(2 * geo.PI + 4.0, True) == pytest.approx(1 / 3)
+
+
+def test_plan_methods_refuse_buffers_of_another_shape():
+    """Every Plan method checks its tensors against the plan's geometry before
+    any device call (the kernels size their launches from the geometry)."""
+    import torch
+    from reconstruction_program_taichi_amd.device import Plan
+    from tests._cases import cone_case
+
+    spec = cone_case(nx=16, nz=8, nv=12, nu=20, nrows=10)
+    plan = Plan.__new__(Plan)
+    plan.spec, plan.h, plan.ctx = spec, None, None
+    sgm_ok = torch.zeros((10, 12, 20))
+    vol_ok = torch.zeros((8, 16, 16))
+    bad_sgm = [torch.zeros((10, 12, 19)), torch.zeros((12, 10, 20)), torch.zeros((10, 11, 20))]
+    bad_vol = [torch.zeros((8, 16, 15)), torch.zeros((16, 16, 8))]
+    for b in bad_sgm:
+        with pytest.raises(ValueError, match="shape"):
+            plan.weight(b)
+        with pytest.raises(ValueError, match="shape"):
+            plan.forward_project(vol_ok, b)
+        with pytest.raises(ValueError, match="shape"):
+            plan.backproject_ir(b, None, vol_ok)
+        with pytest.raises(ValueError, match="shape"):
+            plan.ray_mask(b)
+    for b in bad_vol:
+        with pytest.raises(ValueError, match="shape"):
+            plan.forward_project(b, sgm_ok)
+        with pytest.raises(ValueError, match="shape"):
+            plan.backproject_ir(sgm_ok, None, b)
+    with pytest.raises(ValueError, match="shape"):
+        plan.forward_project(vol_ok, sgm_ok, ray_mask=torch.zeros((10, 12, 10)))
+    with pytest.raises(ValueError, match="multiple"):
+        plan.forward_project(vol_ok, sgm_ok, oversample=3)
+    with pytest.raises(ValueError, match="shape"):
+        plan.forward_project(vol_ok, sgm_ok, oversample=2)  # out must have n_u / 2 columns
+    small = torch.zeros(100, dtype=torch.uint8)
+    with pytest.raises(ValueError, match="bytes"):
+        plan.backproject_fbp(small, "f32", "quad", vol_ok)
+    with pytest.raises(ValueError, match="views"):
+        plan.weight_views(torch.zeros((10, 5, 20)), 9)
+    with pytest.raises(ValueError, match="views"):
+        plan.backproject_views(small, "f32", "quad", 10, 4, vol_ok)
+    with pytest.raises(ValueError, match="shape"):
+        plan.weight_filter(torch.zeros((10, 4, 20)), torch.zeros((10, 4, 21)), np.zeros(39), 1.0, "bf16x3")
+    # right shapes on the host: refused as host memory, still before any call
+    with pytest.raises(ValueError, match="device"):
+        plan.weight(sgm_ok)
+    with pytest.raises(ValueError, match="float32"):
+        plan.ray_mask(torch.zeros((10, 12, 20), dtype=torch.float64))
+
+
+@pytest.mark.parametrize("n", [1, 2, 37, 800])
+def test_ramp_filter_difference_form_is_exact(n):
+    """The MFMA ramp filter's operands: D_m = x_m - x_{m-1} for m = 0..n
+    (x_{-1} = x_n = 0) and G[i] = taps[0] + ... + taps[i-1] (2n entries,
+    G[0] = 0); sum_{m<=n} D_m G[k + n - m] equals the reference's
+    sum_{m<n} x_m taps[k + n-1 - m] (summation by parts), in FP64."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 50.0
+    taps = rng.standard_normal(2 * n - 1)
+    D = np.diff(np.concatenate([[0.0], x, [0.0]]))  # D_0 .. D_n
+    G = np.concatenate([[0.0], np.cumsum(taps)])
+    k = np.arange(n)
+    y = np.array([sum(D[m] * G[kk + n - m] for m in range(n + 1)) for kk in k])
+    want = np.convolve(x, taps)[n - 1:2 * n - 1]
+    assert np.allclose(y, want, rtol=1e-10, atol=1e-9 * np.abs(want).max())
