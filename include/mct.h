@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 6
+#define MCT_ABI_VERSION 7
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -369,6 +369,16 @@ int mct_backproject_ir(mct_plan *plan, const float *sgm, const float *ray_mask, 
  * reduction tree independent of the stream's other work).
  * mct_dot_f64: out[0] = sum a[i]*b[i]. */
 int mct_dot_f64(mct_ctx *ctx, const float *a, const float *b, int64_t n, double *out);
+
+/* mct_dot_f64 without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving or iteration loops, harnesses that forbid
+ * hipMalloc).  Same kernels and result as mct_dot_f64 (bit for bit); its 1024
+ * FP64 partial sums live in a static 8 KiB device array instead of the
+ * context's lazily allocated buffer, so any n fits.  Every call on one
+ * device shares that array: calls that overlap on one device (several
+ * streams or contexts) race and must be serialised by the caller. */
+int mct_dot_f64_noalloc(mct_ctx *ctx, const float *a, const float *b, int64_t n, double *out);
 
 /* Mgfbp_ir.GenerateWR run_mgfbp_iterative_recon_IRN_CGLS.py:234-243:
  * wr = 1/sqrt(ux^2 + uy^2 + uz^2 + beta) of y = x - prior (prior may be NULL),

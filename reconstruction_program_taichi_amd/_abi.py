@@ -24,7 +24,7 @@ MCT_FILTER_F32_DIRECT, MCT_FILTER_BF16_MFMA, MCT_FILTER_BF16X3_MFMA = 0, 1, 2
 MCT_BP_AUTO, MCT_BP_DIRECT, MCT_BP_LDS = 0, 1, 2
 MCT_FP_AUTO, MCT_FP_DIRECT, MCT_FP_CELLS = 0, 1, 2
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 
 class MctError(RuntimeError):
@@ -111,6 +111,7 @@ SIGNATURES = {
     "mct_helical_rebin": (C.c_int, [_P, C.POINTER(Helical), _P, _P, _I32, _P]),
     "mct_backproject_ir": (C.c_int, [_P, _P, _P, _P]),
     "mct_dot_f64": (C.c_int, [_P, _P, _P, _I64, _P]),
+    "mct_dot_f64_noalloc": (C.c_int, [_P, _P, _P, _I64, _P]),
     "mct_tv_weight": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _F64, _P]),
     "mct_dtwd": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _F32, _P, _P]),
     "mct_axpby": (C.c_int, [_P, _F32, _P, _F32, _P, _P, _I64]),

@@ -168,6 +168,8 @@ int mct_create(int device, mct_ctx **out) {
     MCT_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     MCT_HIP(hipGetDeviceProperties(&prop, device));
+    const int rc = mct::noalloc_scratch_init(device);
+    if (rc) return rc;
     mct_ctx *c = new mct_ctx();
     c->device = device;
     c->cu_count = prop.multiProcessorCount;

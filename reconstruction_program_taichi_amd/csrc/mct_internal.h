@@ -142,6 +142,9 @@ int set_error(int code, const char *fmt, ...);
     } while (0)
 
 int ensure_partials(mct_ctx *ctx, int64_t n);
+// Takes the device address of mct_dot_f64_noalloc's static scratch on the
+// current device (once per device; mct_create calls it).
+int noalloc_scratch_init(int device);
 
 // FNV-1a hash of a byte range (tap-set identity for the upload caches).
 inline uint64_t fnv1a(const void *p, size_t n, uint64_t h = 1469598103934665603ull) {

@@ -9,6 +9,8 @@
 // depend on what else runs on the device.
 #include "mct_internal.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kReduceBlocks = 1024;
@@ -188,6 +190,40 @@ int finish(mct_ctx *ctx, double *out) {
 
 }  // namespace
 
+// Scratch of mct_dot_f64_noalloc: the kReduceBlocks FP64 partial sums of one
+// reduction (8 KiB), the same for every length, so no input is too large for
+// it.  Each device holds its own copy for the life of the process; host code
+// uses the device address taken once per device by noalloc_scratch_init,
+// never the symbol itself.  Every call writes all the partials before the
+// finishing kernel reads them, so nothing has to start at zero.
+static __device__ __align__(256) double scratch[kReduceBlocks];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace mct {
+
+// Once per device, with that device current (mct_create calls it).
+int noalloc_scratch_init(int device) {
+    MCT_CHECK(device >= 0 && device < kMaxDevices, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MCT_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(MCT_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                         hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return MCT_OK;
+}
+
+}  // namespace mct
+
 extern "C" {
 
 int mct_dot_f64(mct_ctx *ctx, const float *a, const float *b, int64_t n, double *out) {
@@ -199,6 +235,24 @@ int mct_dot_f64(mct_ctx *ctx, const float *a, const float *b, int64_t n, double 
                        ctx->partials);
     MCT_LAUNCHED();
     return finish(ctx, out);
+}
+
+int mct_dot_f64_noalloc(mct_ctx *ctx, const float *a, const float *b, int64_t n, double *out) {
+    MCT_CHECK(ctx && a && b && out, "NULL argument");
+    MCT_CHECK(n >= 0, "negative length");
+    MCT_CHECK(ctx->device >= 0 && ctx->device < kMaxDevices, "device %d beyond %d", ctx->device,
+              kMaxDevices);
+    double *part;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        part = g_scratch_ptr[ctx->device];
+    }
+    MCT_CHECK(part, "no-allocation scratch is not set up on device %d", ctx->device);
+    hipLaunchKernelGGL(dot_kernel, dim3(kReduceBlocks), dim3(kThreads), 0, ctx->stream, a, b, n, part);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, part, kReduceBlocks,
+                       out);
+    MCT_LAUNCHED();
+    return MCT_OK;
 }
 
 int mct_sq_diff_f64(mct_ctx *ctx, const float *a, const float *b, const float *mb, int64_t n,
