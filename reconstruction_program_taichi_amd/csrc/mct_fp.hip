@@ -128,6 +128,39 @@ __device__ __forceinline__ void clip_range(const FpArgs &a, const Ray &r, float 
     if (k_hi < k_lo) k_hi = k_lo;
 }
 
+// Step range [v_lo, v_hi) that holds every sample of every ray of the view
+// inside the volume's bounding box (same margins as clip_range): from the
+// distance between the source (the rays' common origin (ax, ay, az)) and the
+// box to the distance of its farthest corner.  It depends only on the view,
+// so parts of it taken as segments split every ray of the view at the same
+// steps, whatever rays share a wavefront.
+__device__ __forceinline__ void view_range(const FpArgs &a, const Ray &r, float z0, bool cone,
+                                           int &v_lo, int &v_hi) {
+    const double bx0 = a.x0, bx1 = (double)a.x0 + (double)(a.nx - 1) * a.pix;
+    const double by0 = (double)a.y0 - (double)(a.nx - 1) * a.pix, by1 = a.y0;
+    const double bz0 = z0, bz1 = cone ? (double)z0 + (double)(a.nz - 1) * a.vh : (double)z0;
+    const double sx = r.ax, sy = r.ay, sz = cone ? (double)r.az : bz0;
+    auto near1 = [](double p, double b0, double b1) { return p < b0 ? b0 - p : (p > b1 ? p - b1 : 0.0); };
+    auto far1 = [](double p, double b0, double b1) { return fmax(fabs(p - b0), fabs(p - b1)); };
+    const double dn = sqrt(near1(sx, bx0, bx1) * near1(sx, bx0, bx1) + near1(sy, by0, by1) * near1(sy, by0, by1) +
+                           near1(sz, bz0, bz1) * near1(sz, bz0, bz1));
+    const double df = sqrt(far1(sx, bx0, bx1) * far1(sx, bx0, bx1) + far1(sy, by0, by1) * far1(sy, by0, by1) +
+                           far1(sz, bz0, bz1) * far1(sz, bz0, bz1));
+    // sample k sits at distance l_k from the source along a unit direction;
+    // the P-matrix rays' parameter is not a distance, so they take every step
+    const double dl = (double)a.step * (double)a.diag;
+    double klo = floor((dn - a.lmin) / dl) - 1.0, khi = ceil((df - a.lmin) / dl) + 2.0;
+    if (a.apply_pm) {
+        klo = 0.0;
+        khi = (double)a.count;
+    }
+    klo = klo < 0.0 ? 0.0 : klo;
+    khi = khi > (double)a.count ? (double)a.count : khi;
+    v_lo = (int)klo;
+    v_hi = (int)khi;
+    if (v_hi < v_lo) v_hi = v_lo;
+}
+
 // Line integral of one ray: run_mgfpj.py:554-605.
 __device__ float trace(const FpArgs &a, int ui, int vi, int j) {
     const Ray r = make_ray(a, ui, vi, j);
@@ -191,75 +224,66 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Cell forward projector.  What limits the direct kernel (measured with
-// rocprofv3 counters at 512 x 512 x 256, 108 views of 1024 x 256):
-//  * the vector L1 serves a load four lanes at a time, so a 4-byte gather
-//    costs it as much as a 16-byte one: eight scalar taps per sample keep
-//    it busy with little data;
-//  * a wavefront's 64 rays (neighbouring detector columns) step along the
-//    view's detector direction, which runs along x for some views and along
-//    y for others; along y every lane lands on its own cache line;
-//  * each ray enters the volume at its own step, so at a given iteration the
-//    lanes sit at different depths, spread over rows;
-//  * with the views as the slowest grid dimension, every view streams the
-//    whole volume from HBM.
-// Here the volume is first repacked into cells of bilinear coefficients:
-// cell (z, y, x) = {a, b-a, c-a, d-c-b+a} with a = v[z][y][x],
-// b = v[z][y][x+1], c = v[z][y+1][x], d = v[z][y+1][x+1] (FP64, rounded once)
-// for x, y in [0, nx-2].  A cone-beam record holds the cells of planes z and
-// z+1 (32 B, z in [0, nz-2]), a fan-beam record one plane (16 B).  Records
-// are written in two orders, x-major [z][y][x] and y-major [z][x][y], and
-// each view reads the order whose fast axis follows its detector rows: a
-// sample is then one or two 16-byte loads, and a wavefront's loads touch a
-// few cache lines.  All lanes of a wavefront walk the union of their clipped
-// step ranges, so they sample at the same distance from the source.  The grid
-// runs detector-row bands slowest and views next, so the views in flight read
-// the same slab of the volume, which stays in the last-level cache.  A cell
-// exists exactly when the reference's in-bounds test (index and index + 1
-// inside) passes; out-of-bounds samples read a clamped cell and add 0.
+// Record forward projector.  What limits a one-thread-per-ray gather of the
+// volume (measured with rocprofv3 counters at 512 x 512 x 256, 108 views of
+// 1024 x 256): eight scalar taps per sample keep the vector L1 busy with
+// little data, and 64 rays of neighbouring detector columns spread their
+// samples along the view's detector direction, which for oblique views
+// crosses a new volume row every cell: about 30 cache lines per wavefront
+// load.
+// Here the volume is first repacked into records of bilinear coefficients:
+// record (y, x, z) = {a, b-a, c-a, d-c-b+a} of plane z with a = v[z][y][x],
+// b = v[z][y][x+1], c = v[z][y+1][x], d = v[z][y+1][x+1] (FP64, rounded
+// once), x, y in [0, nx-2], stored z fastest: [y][x][z], 16 B each.  A
+// cone-beam sample reads records z and z+1 (two 16-byte loads, adjacent); a
+// fan-beam sample of detector row (= slice) r reads record r.  A wavefront
+// traces a tile of 8 detector rows x 8 columns (cone; fan: the rows are
+// slices, up to 64 of them per wavefront): at a given step the 8 rows of a
+// column sit at neighbouring z (or are the same ray in neighbouring
+// slices), so their records share cache lines whatever the view angle.  A
+// lane whose sample falls in the cell its previous sample read skips the
+// load and reuses those records (about half of the samples at step 0.2 of
+// the voxel diagonal).  All lanes of a wavefront walk the union of their
+// step ranges, so they sample at the same distance from the source; a
+// sample outside a lane's own range or the volume reads past the end of the
+// record buffer, which returns zeros, and adds +0.  Workgroups (four tiles
+// stacked along the rows) run column tiles fastest, then views, then bands
+// of rows, remapped so each XCD takes a contiguous part of that order: the
+// views in flight on an XCD read one z-range of the records, which its L2
+// and the last-level cache hold.  A record exists exactly when the
+// reference's in-bounds test (index and index + 1 inside) passes.
 
 __device__ __forceinline__ float4 cell_coef(float a, float b, float c, float d) {
     const double A = a, B = b, C = c, D = d;
     return make_float4(a, (float)(B - A), (float)(C - A), (float)(D - C - B + A));
 }
 
-// One 16 x 16 tile of cells of one record plane z per workgroup; both orders
-// written with coalesced stores.
-template <bool CONE>
-__global__ __launch_bounds__(256) void fp_pack_kernel(const float *__restrict__ vol, int nx,
-                                                      float4 *__restrict__ xm,
-                                                      float4 *__restrict__ ym) {
-    constexpr int P = CONE ? 2 : 1;  // voxel planes per record
-    __shared__ float tile[P][17][17];
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16, z = blockIdx.z;
+// Records of 16 cells along x of one row y and 16 planes per workgroup: the
+// volume is read x fastest and the records written z fastest, through LDS;
+// rec_y (optional) receives the same records in the order [x][y][z].
+__global__ __launch_bounds__(256) void fp_pack_kernel(const float *__restrict__ vol, int nx, int nz,
+                                                      float4 *__restrict__ rec,
+                                                      float4 *__restrict__ rec_y) {
+    __shared__ float row0[16][17], row1[16][17];  // [plane][x] of rows y and y + 1
+    const int x0 = blockIdx.x * 16, y = blockIdx.y, z0 = blockIdx.z * 16;
     const int t = threadIdx.x;
     const size_t plane = (size_t)nx * nx;
-    for (int i = t; i < P * 289; i += 256) {
-        const int p = i / 289, r = (i % 289) / 17, c = i % 17;
-        const int gx = x0 + c, gy = y0 + r;
-        tile[p][r][c] = (gx < nx && gy < nx) ? vol[(size_t)(z + p) * plane + (size_t)gy * nx + gx] : 0.f;
+    for (int i = t; i < 2 * 16 * 17; i += 256) {
+        const int r = i / (16 * 17), e = i - r * (16 * 17), zz = e / 17, xx = e - zz * 17;
+        const int gx = x0 + xx, gz = z0 + zz;
+        const float v = (gx < nx && gz < nz) ? vol[(size_t)gz * plane + (size_t)(y + r) * nx + gx] : 0.f;
+        if (r)
+            row1[zz][xx] = v;
+        else
+            row0[zz][xx] = v;
     }
     __syncthreads();
     const int nc = nx - 1;
-    {
-        const int ty = t >> 4, tx = t & 15, cx = x0 + tx, cy = y0 + ty;
-        if (cx < nc && cy < nc) {
-            const size_t o = (((size_t)z * nc + cy) * nc + cx) * P;
-#pragma unroll
-            for (int p = 0; p < P; ++p)
-                xm[o + p] = cell_coef(tile[p][ty][tx], tile[p][ty][tx + 1], tile[p][ty + 1][tx],
-                                      tile[p][ty + 1][tx + 1]);
-        }
-    }
-    {
-        const int tx = t >> 4, ty = t & 15, cx = x0 + tx, cy = y0 + ty;
-        if (cx < nc && cy < nc) {
-            const size_t o = (((size_t)z * nc + cx) * nc + cy) * P;
-#pragma unroll
-            for (int p = 0; p < P; ++p)
-                ym[o + p] = cell_coef(tile[p][ty][tx], tile[p][ty][tx + 1], tile[p][ty + 1][tx],
-                                      tile[p][ty + 1][tx + 1]);
-        }
+    const int xx = t >> 4, zz = t & 15, cx = x0 + xx, cz = z0 + zz;
+    if (cx < nc && cz < nz) {
+        const float4 c = cell_coef(row0[zz][xx], row0[zz][xx + 1], row1[zz][xx], row1[zz][xx + 1]);
+        rec[((size_t)y * nc + cx) * nz + cz] = c;
+        if (rec_y) rec_y[((size_t)cx * nc + y) * nz + cz] = c;
     }
 }
 
@@ -276,7 +300,13 @@ __device__ __forceinline__ int floor_to_int(float x) {
     return r;
 }
 
-__device__ __forceinline__ float4 load_cell(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+// c ? a : b per component (selects on values, never on addresses, so the
+// records of a group stay in registers).
+__device__ __forceinline__ float4 select4(bool c, float4 a, float4 b) {
+    return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
+
+__device__ __forceinline__ float4 load_rec(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
@@ -297,22 +327,25 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Line integral of one ray from cells (same samples, tests and order of
-// summation as trace()).  rsrc covers one order of records; (sx, sy) are the
-// record strides of x and y in it, pz the stride of z (cone) or the fan
-// slice's first record.  Every lane of the wavefront must call this (active =
-// false: a lane without a ray).  Samples go four at a time, loads first.
+// A byte offset past every record: the buffer holds fewer than 2^32 - 64
+// bytes, so this offset and the one 16 bytes after it both read zeros.
+constexpr uint32_t kRecOut = 0u - 32u;
+
+// Line integral of one ray from the records (the same samples, tests and
+// order of summation as trace()).  [k_lo, k_hi) is the ray's clipped step
+// range; with nseg > 1 the lane takes the steps of part seg of nseg of its
+// view's step range (view_range: the same parts for every ray of the view,
+// so the partial sums of a ray do not depend on the other rays of its
+// wavefront).  Every lane of the wavefront must call this (active = false:
+// a lane without a ray).  Samples go four at a time, loads first.
 //
 // The sample distance l of step k comes from ltab (the plan's table in the
 // same FP32 expression order); k is wave-uniform, so it is a scalar load.
-// A sample outside the volume loads from past the end of rsrc, which returns
-// zeros, so it adds +0; inside, the indices are those of the reference's
-// in-bounds test.  pz (cone) is below 2^24; the fan's record offsets are
-// formed in bytes (two 24-bit multiply-adds).
-template <bool CONE>
-__device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bool active, int ui,
-                             int vi, int j, __amdgpu_buffer_rsrc_t rsrc, uint32_t sx, uint32_t sy,
-                             uint32_t pz, int seg, int nseg) {
+// (sxb, syb): byte strides of x and y in the record order rsrc covers.
+template <bool CONE, bool REUSE>
+__device__ float trace_rec(const FpArgs &a, const float *__restrict__ ltab, bool active, int ui,
+                           int vi, int j, __amdgpu_buffer_rsrc_t rsrc, uint32_t sxb, uint32_t syb,
+                           int seg, int nseg) {
     const Ray r = make_ray(a, active ? ui : 0, vi, j);
     const float z0 = a.zbase + (a.center_z ? a.center_z[j] : a.cz);
     int k_lo, k_hi;
@@ -321,24 +354,25 @@ __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bo
         k_lo = 0x7fffffff;
         k_hi = -1;
     }
-    k_lo = __builtin_amdgcn_readfirstlane(wave_min(k_lo));
-    k_hi = __builtin_amdgcn_readfirstlane(wave_max(k_hi));
-    if (nseg > 1 && k_hi > k_lo) {  // this wavefront's share of the steps
-        const int64_t len = (int64_t)k_hi - k_lo;
-        const int s0 = k_lo + (int)(len * seg / nseg), s1 = k_lo + (int)(len * (seg + 1) / nseg);
-        k_lo = s0;
-        k_hi = s1;
+    int w_lo = __builtin_amdgcn_readfirstlane(wave_min(k_lo));
+    int w_hi = __builtin_amdgcn_readfirstlane(wave_max(k_hi));
+    if (nseg > 1) {  // part seg of the view's step range
+        int v_lo, v_hi;
+        view_range(a, r, z0, CONE, v_lo, v_hi);
+        const int64_t len = (int64_t)v_hi - v_lo;
+        const int s0 = v_lo + (int)(len * seg / nseg), s1 = v_lo + (int)(len * (seg + 1) / nseg);
+        w_lo = w_lo > s0 ? w_lo : s0;
+        w_hi = w_hi < s1 ? w_hi : s1;
     }
     const float dl = a.step * a.diag;
-    const int nc = a.nx - 1;
-    constexpr uint32_t kRec = CONE ? 32u : 16u;
-    // byte offset past every record (the orders hold under 4 GiB, a multiple of kRec)
-    constexpr uint32_t kOut = 0u - kRec;
-    const uint32_t bsx = sx * kRec, bsy = sy * kRec, bpz = pz * kRec;
+    const uint32_t nc = (uint32_t)(a.nx - 1), nzr = (uint32_t)a.nz;
+    const uint32_t slice = CONE ? 0u : (uint32_t)vi * 16u;
     float sum = 0.f;
+    uint32_t prev = 0xffffffffu;  // offset of the records last loaded (none yet)
+    float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
     constexpr int U = 4;
-    for (int k0 = k_lo; k0 < k_hi; k0 += U) {
-        float4 q[U][CONE ? 2 : 1];
+    for (int k0 = w_lo; k0 < w_hi; k0 += U) {
+        uint32_t off[U];
         float xw[U], yw[U], zw[U];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
@@ -349,31 +383,59 @@ __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bo
             const float fx = (xr - a.x0) * a.inv_pix;
             const float fy = (yr - a.y0) * (-a.inv_pix);
             const int xi = floor_to_int(fx), yi = floor_to_int(fy);
-            bool in = active && k < k_hi && (uint32_t)xi < (uint32_t)nc && (uint32_t)yi < (uint32_t)nc;
+            // k < w_hi: the last group may run past the wavefront's (or the
+            // segment's) end.  A step outside the lane's own clipped range lies
+            // outside the volume's box by at least a step, so the index test
+            // rejects it.
+            bool in = k < w_hi && (uint32_t)xi < nc && (uint32_t)yi < nc;
             xw[i] = __builtin_amdgcn_fractf(fx);
             yw[i] = __builtin_amdgcn_fractf(fy);
+            uint32_t o = __umul24((uint32_t)yi, syb) + __umul24((uint32_t)xi, sxb);
             if (CONE) {
                 const float zr = fmaf(l, r.bz, r.az);
                 const float fz = (zr - z0) * a.inv_vh;
                 const int zi = floor_to_int(fz);
-                in = in && (uint32_t)zi < (uint32_t)(a.nz - 1);
+                in = in && (uint32_t)zi < nzr - 1u;
                 zw[i] = __builtin_amdgcn_fractf(fz);
-                const uint32_t cell =
-                    __umul24((uint32_t)xi, sx) + __umul24((uint32_t)yi, sy) + __umul24((uint32_t)zi, pz);
-                const uint32_t off = in ? cell * kRec : kOut;
-                q[i][0] = load_cell(rsrc, off);
-                q[i][CONE ? 1 : 0] = load_cell(rsrc, off + 16u);  // kOut + 16 is past the end too
+                o += (uint32_t)zi * 16u;
             } else {
-                const uint32_t off = __umul24((uint32_t)xi, bsx) + (__umul24((uint32_t)yi, bsy) + bpz);
-                q[i][0] = load_cell(rsrc, in ? off : kOut);
+                zw[i] = 0.f;
+                o += slice;
             }
+            off[i] = in ? o : kRecOut;
         }
-        // an outside sample's cell is zeros and its weights lie in [0, 1): it adds +0
+        // REUSE: a lane whose sample reads the records of its previous sample
+        // points its load past the end of the buffer (range-checked: no cache
+        // access) and takes the previous records instead, so every load of
+        // the group is issued before any is waited for.  It pays where the
+        // lanes of a wavefront share a ray (then they reuse together).  An
+        // outside sample reads zeros (weights in [0, 1): +0).
+        float4 q0[U], q1[U];
+        bool same[U];
+        uint32_t last = prev;
 #pragma unroll
         for (int i = 0; i < U; ++i) {
-            float val = cell_lerp(q[i][0], xw[i], yw[i]);
+            same[i] = REUSE && off[i] == last;
+            last = off[i];
+            const uint32_t o = same[i] ? kRecOut : off[i];
+            q0[i] = load_rec(rsrc, o);
+            if (CONE) q1[i] = load_rec(rsrc, o + 16u);
+        }
+        if (REUSE) {
+#pragma unroll
+            for (int i = 0; i < U; ++i) {
+                q0[i] = select4(same[i], i ? q0[i - 1] : p0, q0[i]);
+                if (CONE) q1[i] = select4(same[i], i ? q1[i - 1] : p1, q1[i]);
+            }
+            p0 = q0[U - 1];
+            if (CONE) p1 = q1[U - 1];
+            prev = off[U - 1];
+        }
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            float val = cell_lerp(q0[i], xw[i], yw[i]);
             if (CONE) {
-                const float hi = cell_lerp(q[i][CONE ? 1 : 0], xw[i], yw[i]);
+                const float hi = cell_lerp(q1[i], xw[i], yw[i]);
                 val = fmaf(zw[i], hi - val, val);
             }
             sum += val;
@@ -382,68 +444,81 @@ __device__ float trace_cells(const FpArgs &a, const float *__restrict__ ltab, bo
     return sum * dl;
 }
 
-// fp_kernel on cells; xm / ym: the two record orders, `bytes` each (under
-// 4 GiB).  One wavefront traces 64 neighbouring columns of one detector row
-// and view.  rows4 = 1 (four or more rows): a workgroup's four wavefronts take
-// four neighbouring rows of the same columns and view, workgroups numbered
-// column block fastest, then view, then band of four rows; rows4 = 0: the
+// fp_kernel on records (`bytes` of them, under 2^32 - 64).  A wavefront
+// traces tr neighbouring detector rows x (64 / tr) neighbouring columns of
+// one view (lanes row fastest).  wr = 4: a workgroup holds four such tiles
+// stacked along the rows, and workgroups are numbered column block fastest,
+// then view, then band of 4 tr rows; wr = 1 (rows too few for that): the
 // wavefronts themselves are numbered column block fastest, then view, then
-// row, so a single-row (fan slice) problem still fills every wavefront.
-// nwaves counts the numbered wavefronts (idle ones of a partial band
-// included); each segment of a split ray range repeats the numbering.
-template <bool CONE>
-__global__ __launch_bounds__(256) void fp_cells_kernel(FpArgs a, const float *__restrict__ ltab,
-                                                       const float4 *xm, const float4 *ym,
-                                                       uint32_t bytes, int nub, int nwaves, int nseg,
-                                                       int rows4, float *partial) {
-    // wave-uniform by construction; readfirstlane keeps the decode below and
-    // the per-view loads it feeds in scalar registers
-    const int wg = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (wg >= nwaves * nseg) return;  // a whole wavefront
-    const int seg = wg / nwaves, w = wg - seg * nwaves;
-    int ub, j, vi;
-    if (rows4) {
-        const int q = w >> 2, t = q / nub;
+// row tile, four to a workgroup.  Each segment of a split ray range repeats
+// the numbering; the whole is remapped so that each XCD takes a contiguous
+// part (nwg: workgroups of one segment; the grid is padded to a multiple of
+// the XCD count).
+template <bool CONE, bool REUSE>
+__device__ __forceinline__ void fp_rec_body(const FpArgs &a, const float *__restrict__ ltab,
+                                            const float4 *rec, const float4 *rec_y, uint32_t bytes,
+                                            int tr, int wr, int nub, int nwg, int nseg, int nxcd,
+                                            float *partial) {
+    const int g = mct::xcd_remap((int)blockIdx.x, (int)gridDim.x, nxcd);
+    if (g >= nwg * nseg) return;  // padding
+    const int seg = g / nwg, q = g - seg * nwg;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int tc = 64 / tr;
+    int ub, j, rt;  // column block, view and row tile of this wavefront
+    if (wr == 4) {  // the four tiles stacked along the rows
+        const int t = q / nub;
         ub = q % nub;
         j = t % a.nviews;
-        vi = (t / a.nviews) * 4 + (w & 3);
-        if (vi >= a.nrows) return;  // a whole wavefront
-    } else {
-        const int t = w / nub;
+        rt = (t / a.nviews) * 4 + wave;
+    } else {  // wavefronts numbered one by one (column block fastest, then view, then row tile)
+        const int w = q * 4 + wave, t = w / nub;
         ub = w % nub;
         j = t % a.nviews;
-        vi = t / a.nviews;
+        rt = t / a.nviews;
     }
-    const int uo = ub * 64 + (threadIdx.x & 63);
-    const bool inside = uo < a.nu_out;
-    const size_t oidx = ((size_t)vi * a.nviews + j) * a.nu_out + (inside ? uo : 0);
+    if (j >= a.nviews) return;  // a whole wavefront past the last one (wr = 1)
+    const int vi = rt * tr + (lane % tr);
+    const int uo = ub * tc + lane / tr;
+    const bool inside = uo < a.nu_out && vi < a.nrows;
+    const size_t oidx = inside ? ((size_t)vi * a.nviews + j) * a.nu_out + uo : 0;
     const bool masked = inside && a.mask && a.mask[oidx] != 1.0f;
     const bool active = inside && !masked;
-    // neighbouring detector columns step along (-sin, cos) of the view in
-    // the volume (P-matrix: along the rotated first column of A): the
-    // y-major order when that is closer to the y axis
-    bool ymaj;
-    if (!a.apply_pm) {
-        const float2 cs = a.ft[j];
-        ymaj = fabsf(cs.x) >= fabsf(cs.y);
-    } else {
-        const float *A = a.pm_a + 9 * j;
-        ymaj = fabsf(A[3] * a.c0 + A[0] * a.s0) >= fabsf(A[0] * a.c0 - A[3] * a.s0);
+    // with a second order (few slices per tile, so a tile's lanes run along
+    // the detector columns), the one whose fast axis follows the view's
+    // detector direction (-sin, cos) (P-matrix: the rotated first column of A)
+    bool ymaj = false;
+    if (rec_y) {
+        if (!a.apply_pm) {
+            const float2 cs = a.ft[j];
+            ymaj = fabsf(cs.x) >= fabsf(cs.y);
+        } else {
+            const float *A = a.pm_a + 9 * j;
+            ymaj = fabsf(A[3] * a.c0 + A[0] * a.s0) >= fabsf(A[0] * a.c0 - A[3] * a.s0);
+        }
     }
-    const uint32_t nc = (uint32_t)(a.nx - 1);
-    const uint32_t sx = ymaj ? nc : 1u, sy = ymaj ? 1u : nc;
-    const uint32_t pz = CONE ? nc * nc : (uint32_t)vi * nc * nc;
+    const uint32_t colb = (uint32_t)a.nz * 16u, rowb = (uint32_t)(a.nx - 1) * colb;
     const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void *)(ymaj ? ym : xm), 0, (int)bytes, kBufferDword3);
+        __builtin_amdgcn_make_buffer_rsrc((void *)(ymaj ? rec_y : rec), 0, (int)bytes, kBufferDword3);
+    const uint32_t sxb = ymaj ? rowb : colb, syb = ymaj ? colb : rowb;
     float acc = 0.f;
     for (int i = 0; i < a.os; ++i)
-        acc += trace_cells<CONE>(a, ltab, active, uo * a.os + i, vi, j, rsrc, sx, sy, pz, seg, nseg);
+        acc += trace_rec<CONE, REUSE>(a, ltab, active, uo * a.os + i, inside ? vi : 0, j, rsrc, sxb,
+                                      syb, seg, nseg);
     if (!inside) return;
     const float v = masked ? 0.f : acc / (float)a.os;
     if (nseg == 1)
         a.out[oidx] = v;
     else  // partial sum of segment seg; fp_segments_kernel adds them
         partial[(size_t)seg * a.nrows * a.nviews * a.nu_out + oidx] = v;
+}
+
+template <bool CONE, bool REUSE>
+__global__ __launch_bounds__(256) void fp_rec_kernel(FpArgs a, const float *__restrict__ ltab,
+                                                     const float4 *rec, const float4 *rec_y,
+                                                     uint32_t bytes, int tr, int wr, int nub, int nwg,
+                                                     int nseg, int nxcd, float *partial) {
+    fp_rec_body<CONE, REUSE>(a, ltab, rec, rec_y, bytes, tr, wr, nub, nwg, nseg, nxcd, partial);
 }
 
 // out[i] = sum of the nseg partial sums of ray i, in segment order.
@@ -673,58 +748,68 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
     mct_ctx *ctx = plan->ctx;
     dim3 grid((a.nu_out + 63) / 64, (a.nrows + 3) / 4, a.nviews);
     if (ctx->fp_kernel != MCT_FP_DIRECT) {
-        const int64_t nx = g.img_dim;
-        // records per order: cone (nz-1) planes of 32 B, fan nz planes of 16 B
+        const int64_t nx = g.img_dim, nz = g.img_dim_z;
         const int64_t nc = nx - 1;
-        const int64_t planes = g.cone ? (int64_t)g.img_dim_z - 1 : (int64_t)g.img_dim_z;
-        const int64_t bytes = nc * nc * planes * (g.cone ? 32 : 16);
-        const int64_t nub = (a.nu_out + 63) / 64;
-        const int rows4 = a.nrows >= 4;
-        const int64_t nwaves = nub * a.nviews * (rows4 ? 4 * (((int64_t)a.nrows + 3) / 4) : a.nrows);
+        const int64_t bytes = nc * nc * nz * 16;  // records [y][x][z]
+        // tile rows: cone 8 (x 8 columns); fan up to 64 slices of one ray
+        int tr = 8;
+        if (!g.cone) {
+            tr = 1;
+            while (tr < 64 && tr < a.nrows) tr *= 2;
+        }
+        const int tc = 64 / tr;
+        const int wr = (int64_t)4 * tr <= a.nrows ? 4 : 1;
+        const int64_t nub = (a.nu_out + tc - 1) / tc;
+        const int64_t rtiles = ((int64_t)a.nrows + tr - 1) / tr;
+        const int64_t nwg = wr == 4 ? nub * a.nviews * ((rtiles + 3) / 4)
+                                    : (nub * a.nviews * rtiles + 3) / 4;
         // Small problems (e.g. one fan slice) would not fill the device with
-        // one wavefront per 64 rays: split each ray's steps over nseg <= 16
-        // wavefronts (toward 256 wavefronts per CU in all, so the last round
-        // of wavefronts is a small part of the launch) and add the partial
-        // sums in a second pass.  (512^2 fan slice, 240 views: 0.46 ms with
-        // 32 wavefronts per CU, 0.35 ms with 256.)  The split counts the
-        // wavefronts that carry rays (rows not padded to bands of 4), so a
-        // problem of n rows with the target scaled by n splits each ray
-        // exactly as a one-row problem does.
+        // one lane per ray: split each ray's steps over nseg <= 16 lanes
+        // (toward 256 wavefronts per CU in all, so the last round of
+        // wavefronts is a small part of the launch) and add the partial sums
+        // in a second pass.  (512^2 fan slice, 240 views: 0.46 ms with 32
+        // wavefronts per CU, 0.35 ms with 256.)  The split counts the rays
+        // (wavefronts of 64 rays), so a problem of n rows with the target
+        // scaled by n splits each ray exactly as a one-row problem does; the
+        // parts come from each ray's own step range.
         const int64_t target = (int64_t)ctx->cu_count * (ctx->fp_split ? ctx->fp_split : 256);
-        const int64_t nwaves_rays = nub * a.nviews * a.nrows;
+        const int64_t nwaves_rays = ((int64_t)a.nu_out * a.nrows * a.nviews + 63) / 64;
         int64_t nseg = (target + nwaves_rays - 1) / nwaves_rays;
         nseg = nseg < 1 ? 1 : (nseg > 16 ? 16 : nseg);
-        const int64_t nblocks = (nwaves * nseg + 3) / 4;
-        // 32-bit record offsets; gridDim.z of the pack launch; wavefront index
-        // (and 24-bit plane strides for the cone kernel's address arithmetic)
-        const bool fits = nc >= 1 && planes >= 1 && bytes < ((int64_t)1 << 32) && planes <= 65535 &&
-                          nwaves * nseg < ((int64_t)1 << 31) && nc * nc < ((int64_t)1 << 24);
+        const int nxcd = ctx->xcd_count > 0 ? ctx->xcd_count : 1;
+        const int64_t nblocks = (nwg * nseg + nxcd - 1) / nxcd * nxcd;
+        // 32-bit record offsets (and 24-bit row strides for the address
+        // arithmetic); gridDim.y of the pack launch
+        const bool fits = nc >= 1 && nz >= 1 && bytes < ((int64_t)1 << 32) - 64 &&
+                          nc * nz * 16 < ((int64_t)1 << 24) && nc * nc * nz * 16 < ((int64_t)1 << 32) &&
+                          nblocks < ((int64_t)1 << 31) && nc <= 65535;
         if (fits) {
             int rc = ensure_step_table(plan, a);
             if (rc) return rc;
-            rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, 2 * (bytes / 4));
+            const bool two = !g.cone && tr <= 4;  // a second, y-major order
+            rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, (two ? 2 : 1) * (bytes / 4));
             if (rc) return rc;
             const int64_t nout = (int64_t)a.nrows * a.nviews * a.nu_out;
             if (nseg > 1) {
                 rc = mct::ensure_buffer(ctx, &ctx->fp_partial, &ctx->fp_partial_cap, nseg * nout);
                 if (rc) return rc;
             }
-            float4 *xm = reinterpret_cast<float4 *>(ctx->fp_pack);
-            float4 *ym = reinterpret_cast<float4 *>(ctx->fp_pack + bytes / 4);
-            const dim3 pgrid((unsigned)((nc + 15) / 16), (unsigned)((nc + 15) / 16), (unsigned)planes);
-            if (g.cone) {
-                hipLaunchKernelGGL(fp_pack_kernel<true>, pgrid, dim3(256), 0, ctx->stream, vol,
-                                   g.img_dim, xm, ym);
-                hipLaunchKernelGGL(fp_cells_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, plan->fp_ltab, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
-                                   (int)nseg, rows4, ctx->fp_partial);
-            } else {
-                hipLaunchKernelGGL(fp_pack_kernel<false>, pgrid, dim3(256), 0, ctx->stream, vol,
-                                   g.img_dim, xm, ym);
-                hipLaunchKernelGGL(fp_cells_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0,
-                                   ctx->stream, a, plan->fp_ltab, xm, ym, (uint32_t)bytes, (int)nub, (int)nwaves,
-                                   (int)nseg, rows4, ctx->fp_partial);
-            }
+            float4 *rec = reinterpret_cast<float4 *>(ctx->fp_pack);
+            float4 *rec_y = two ? reinterpret_cast<float4 *>(ctx->fp_pack + bytes / 4) : nullptr;
+            const dim3 pgrid((unsigned)((nc + 15) / 16), (unsigned)nc, (unsigned)((nz + 15) / 16));
+            hipLaunchKernelGGL(fp_pack_kernel, pgrid, dim3(256), 0, ctx->stream, vol, g.img_dim,
+                               g.img_dim_z, rec, rec_y);
+#define FP_REC_LAUNCH(CONE, REUSE)                                                                  \
+    hipLaunchKernelGGL((fp_rec_kernel<CONE, REUSE>), dim3((unsigned)nblocks), dim3(256), 0, ctx->stream, \
+                       a, plan->fp_ltab, rec, rec_y, (uint32_t)bytes, tr, wr, (int)nub, (int)nwg,      \
+                       (int)nseg, nxcd, ctx->fp_partial)
+            if (g.cone)
+                FP_REC_LAUNCH(true, true);
+            else if (tr == 64)  // the lanes of a wavefront are one ray in 64 slices
+                FP_REC_LAUNCH(false, true);
+            else
+                FP_REC_LAUNCH(false, false);
+#undef FP_REC_LAUNCH
             if (nseg > 1)
                 hipLaunchKernelGGL(fp_segments_kernel, dim3(mct::blocks_for(nout, 256)), dim3(256), 0,
                                    ctx->stream, ctx->fp_partial, nout, (int)nseg, out);
@@ -734,8 +819,8 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         }
         if (ctx->fp_kernel == MCT_FP_CELLS)
             return mct::set_error(MCT_ERR_UNSUPPORTED,
-                                  "cell forward projection: one order of cells (%lld B) does not "
-                                  "fit 32-bit offsets",
+                                  "record forward projection: the records (%lld B) do not fit "
+                                  "32-bit offsets",
                                   (long long)bytes);
     }
     hipLaunchKernelGGL(fp_kernel, grid, dim3(256), 0, ctx->stream, a);
