@@ -232,9 +232,9 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 // crosses a new volume row every cell: about 30 cache lines per wavefront
 // load.
 // Here the volume is first repacked into records of bilinear coefficients:
-// record (y, x, z) = {a, b-a, c-a, d-c-b+a} of plane z with a = v[z][y][x],
-// b = v[z][y][x+1], c = v[z][y+1][x], d = v[z][y+1][x+1] (FP64, rounded
-// once), x, y in [0, nx-2], stored z fastest: [y][x][z], 16 B each.  A
+// record (y, x, z) = {a, b-a, c-a, (d-c)-(b-a)} of plane z with
+// a = v[z][y][x], b = v[z][y][x+1], c = v[z][y+1][x], d = v[z][y+1][x+1]
+// (FP32), x, y in [0, nx-2], stored z fastest: [y][x][z], 16 B each.  A
 // cone-beam sample reads records z and z+1 (two 16-byte loads, adjacent); a
 // fan-beam sample of detector row (= slice) r reads record r.  A wavefront
 // traces a tile of 8 detector rows x 8 columns (cone; fan: the rows are
@@ -253,9 +253,11 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 // and the last-level cache hold.  A record exists exactly when the
 // reference's in-bounds test (index and index + 1 inside) passes.
 
+// Bilinear coefficients of a cell, in FP32: v(xw, yw) = a + xw (b-a)
+// + yw ((c-a) + xw ((d-c) - (b-a))).
 __device__ __forceinline__ float4 cell_coef(float a, float b, float c, float d) {
-    const double A = a, B = b, C = c, D = d;
-    return make_float4(a, (float)(B - A), (float)(C - A), (float)(D - C - B + A));
+    const float ba = b - a;
+    return make_float4(a, ba, c - a, (d - c) - ba);
 }
 
 // Records of 16 cells along x of one row y and 16 planes per workgroup: the
@@ -285,6 +287,24 @@ __global__ __launch_bounds__(256) void fp_pack_kernel(const float *__restrict__ 
         rec[((size_t)y * nc + cx) * nz + cz] = c;
         if (rec_y) rec_y[((size_t)cx * nc + y) * nz + cz] = c;
     }
+}
+
+// The volume with the slice index fastest: volt[(y * nx + x) * nz + z] =
+// vol[z][y][x] (16 x positions of one row y and 16 slices per workgroup;
+// read x fastest, written z fastest, through LDS).
+__global__ __launch_bounds__(256) void fp_transpose_kernel(const float *__restrict__ vol, int nx, int nz,
+                                                           float *__restrict__ volt) {
+    __shared__ float tile[16][17];  // [slice][x]
+    const int x0 = blockIdx.x * 16, y = blockIdx.y, z0 = blockIdx.z * 16;
+    const int t = threadIdx.x;
+    const size_t plane = (size_t)nx * nx;
+    {
+        const int zz = t >> 4, xx = t & 15, gx = x0 + xx, gz = z0 + zz;
+        tile[zz][xx] = (gx < nx && gz < nz) ? vol[(size_t)gz * plane + (size_t)y * nx + gx] : 0.f;
+    }
+    __syncthreads();
+    const int xx = t >> 4, zz = t & 15, gx = x0 + xx, gz = z0 + zz;
+    if (gx < nx && gz < nz) volt[((size_t)y * nx + gx) * nz + gz] = tile[zz][xx];
 }
 
 __device__ __forceinline__ float cell_lerp(float4 q, float xw, float yw) {
@@ -519,6 +539,152 @@ __global__ __launch_bounds__(256) void fp_rec_kernel(FpArgs a, const float *__re
                                                      uint32_t bytes, int tr, int wr, int nub, int nwg,
                                                      int nseg, int nxcd, float *partial) {
     fp_rec_body<CONE, REUSE>(a, ltab, rec, rec_y, bytes, tr, wr, nub, nwg, nseg, nxcd, partial);
+}
+
+// Fan beam with at least 64 detector rows (slices): every row of a view and
+// column is the same line (the rows differ only in the slice they sample),
+// so a wavefront traces one ray in 64 slices (lane = slice) and computes the
+// ray's sample positions, weights and in-bounds tests once, 64 steps at a
+// time with a lane per step.  It then walks the steps in order: for each cell
+// the ray enters, the four corner voxels of the 64 slices (256 B each, from
+// the slice-fastest copy of the volume, which the last-level cache holds
+// across views) are loaded once (four cells' loads in flight together) and
+// turned into the cell's bilinear coefficients as the records are; each
+// step's weights go through LDS (written by the step's lane, read by all
+// lanes; one wavefront's LDS operations complete in order).  Same samples, tests,
+// coefficients and order of summation as the record kernel (steps outside
+// the volume add nothing).
+__device__ __forceinline__ float4 corner_coef(float a, float b, float c, float d) {
+    return cell_coef(a, b, c, d);
+}
+
+__device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
+                             __amdgpu_buffer_rsrc_t rsrc, uint32_t slice4, int ui, int j, int seg,
+                             int nseg, float2 *__restrict__ wts) {
+    const Ray r = make_ray(a, ui, 0, j);
+    int k_lo, k_hi;
+    clip_range(a, r, 0.f, false, k_lo, k_hi);
+    if (nseg > 1) {  // part seg of the view's step range
+        int v_lo, v_hi;
+        view_range(a, r, 0.f, false, v_lo, v_hi);
+        const int64_t len = (int64_t)v_hi - v_lo;
+        const int s0 = v_lo + (int)(len * seg / nseg), s1 = v_lo + (int)(len * (seg + 1) / nseg);
+        k_lo = k_lo > s0 ? k_lo : s0;
+        k_hi = k_hi < s1 ? k_hi : s1;
+    }
+    k_lo = __builtin_amdgcn_readfirstlane(k_lo);
+    k_hi = __builtin_amdgcn_readfirstlane(k_hi);
+    const int lane = threadIdx.x & 63;
+    const uint32_t nc = (uint32_t)(a.nx - 1);
+    const uint32_t xb = (uint32_t)a.nz * 4u, yb = (uint32_t)a.nx * xb;  // byte strides of x and y
+    const float dl = a.step * a.diag;
+    float sum = 0.f;
+    // the cell of the last step of the previous chunk (inside or not) and its coefficients
+    uint32_t c_off = kRecOut;
+    int c_in = 0;
+    float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k0 = k_lo; k0 < k_hi; k0 += 64) {
+        const int k = k0 + lane;
+        const float l = ltab[k < a.count ? k : a.count - 1];
+        const float xr = fmaf(l, r.bx, r.ax);
+        const float yr = fmaf(l, r.by, r.ay);
+        const float fx = (xr - a.x0) * a.inv_pix;
+        const float fy = (yr - a.y0) * (-a.inv_pix);
+        const int xi = floor_to_int(fx), yi = floor_to_int(fy);
+        const bool in = k < k_hi && (uint32_t)xi < nc && (uint32_t)yi < nc;
+        // this chunk's weights, read back by every lane (broadcast) step by step
+        __builtin_amdgcn_wave_barrier();
+        wts[lane] = make_float2(__builtin_amdgcn_fractf(fx), __builtin_amdgcn_fractf(fy));
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t off = __umul24((uint32_t)yi, yb) + __umul24((uint32_t)xi, xb);
+        uint32_t p_off = (uint32_t)__shfl_up((int)off, 1, 64);
+        int p_in = __shfl_up((int)in, 1, 64);
+        if (lane == 0) {
+            p_off = c_off;
+            p_in = c_in;
+        }
+        const bool fresh = in && !(p_in && p_off == off);
+        const uint64_t m_in = __builtin_amdgcn_ballot_w64(in);
+        uint64_t m_new = __builtin_amdgcn_ballot_w64(fresh);
+        c_off = (uint32_t)__builtin_amdgcn_readlane((int)off, 63);
+        c_in = (m_in >> 63) & 1;
+        // steps at the start that continue the previous chunk's cell
+        const int first = m_new ? __builtin_ctzll(m_new) : 64;
+        for (int s = 0; s < first && ((m_in >> s) & 1); ++s) {
+            const float2 w = wts[s];
+            sum += cell_lerp(cur, w.x, w.y);
+        }
+        while (m_new) {
+            int n[4];
+            float v[4][4];
+            int cnt = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                n[t] = 64;
+                if (m_new) {
+                    n[t] = __builtin_ctzll(m_new);
+                    m_new &= m_new - 1;
+                    ++cnt;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (t < cnt) {
+                    const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)off, n[t]) + slice4;
+                    v[t][0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o, 0, 0));
+                    v[t][1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o + xb, 0, 0));
+                    v[t][2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o + yb, 0, 0));
+                    v[t][3] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o + yb + xb, 0, 0));
+                }
+            const int tail = m_new ? __builtin_ctzll(m_new) : 64;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (t < cnt) {
+                    const float4 q = corner_coef(v[t][0], v[t][1], v[t][2], v[t][3]);
+                    const int end = t + 1 < cnt ? n[t + 1] : tail;
+                    for (int s = n[t]; s < end && ((m_in >> s) & 1); ++s) {
+                        const float2 w = wts[s];
+                        sum += cell_lerp(q, w.x, w.y);
+                    }
+                    cur = q;
+                }
+            }
+        }
+    }
+    return sum * dl;
+}
+
+// One wavefront per (output column, view, tile of 64 slices), numbered
+// column fastest, then view, then slice tile, four to a workgroup; segments
+// and XCD remapping as in fp_rec_kernel.
+__global__ __launch_bounds__(256) void fp_ray64_kernel(FpArgs a, const float *__restrict__ ltab,
+                                                       const float *volt, uint32_t bytes, int nwg,
+                                                       int nseg, int nxcd, float *partial) {
+    const int g = mct::xcd_remap((int)blockIdx.x, (int)gridDim.x, nxcd);
+    if (g >= nwg * nseg) return;  // padding
+    const int seg = g / nwg, q = g - seg * nwg;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int w = q * 4 + wave, t = w / a.nu_out;
+    const int uo = w - t * a.nu_out, j = t % a.nviews, st = t / a.nviews;
+    if (st * 64 >= a.nrows) return;  // a whole wavefront past the last tile
+    const int vi = st * 64 + lane;
+    const bool inside = vi < a.nrows;
+    const size_t oidx = inside ? ((size_t)vi * a.nviews + j) * a.nu_out + uo : 0;
+    const bool masked = inside && a.mask && a.mask[oidx] != 1.0f;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)volt, 0, (int)bytes, kBufferDword3);
+    const uint32_t slice4 = (uint32_t)(inside ? vi : 0) * 4u;
+    __shared__ float2 wts[4][64];  // per wavefront: the current chunk's step weights
+    float acc = 0.f;
+    for (int i = 0; i < a.os; ++i)
+        acc += trace_ray64(a, ltab, rsrc, slice4, uo * a.os + i, j, seg, nseg, wts[wave]);
+    if (!inside) return;
+    const float v = masked ? 0.f : acc / (float)a.os;
+    if (nseg == 1)
+        a.out[oidx] = v;
+    else  // partial sum of segment seg; fp_segments_kernel adds them
+        partial[(size_t)seg * a.nrows * a.nviews * a.nu_out + oidx] = v;
 }
 
 // out[i] = sum of the nseg partial sums of ray i, in segment order.
@@ -778,6 +944,41 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         nseg = nseg < 1 ? 1 : (nseg > 16 ? 16 : nseg);
         const int nxcd = ctx->xcd_count > 0 ? ctx->xcd_count : 1;
         const int64_t nblocks = (nwg * nseg + nxcd - 1) / nxcd * nxcd;
+        const int64_t nout = (int64_t)a.nrows * a.nviews * a.nu_out;
+        // Fan beam, at least 64 rows, no P-matrix: a wavefront per ray in 64
+        // slices, on the slice-fastest copy of the volume (32-bit byte
+        // offsets, 24-bit row stride).
+        const int64_t tbytes = nx * nx * nz * 4;
+        if (!g.cone && !g.apply_pmatrix && a.nrows >= 64 && tbytes < ((int64_t)1 << 32) - 64 &&
+            nx * nz * 4 < ((int64_t)1 << 24) && nx <= 65535) {
+            const int64_t nw = (int64_t)a.nu_out * a.nviews * (((int64_t)a.nrows + 63) / 64);
+            const int64_t nwg64 = (nw + 3) / 4;
+            int64_t nseg64 = (target + nw - 1) / nw;
+            nseg64 = nseg64 < 1 ? 1 : (nseg64 > 16 ? 16 : nseg64);
+            const int64_t nb64 = (nwg64 * nseg64 + nxcd - 1) / nxcd * nxcd;
+            if (nb64 < ((int64_t)1 << 31)) {
+                int rc = ensure_step_table(plan, a);
+                if (rc) return rc;
+                rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, tbytes / 4);
+                if (rc) return rc;
+                if (nseg64 > 1) {
+                    rc = mct::ensure_buffer(ctx, &ctx->fp_partial, &ctx->fp_partial_cap, nseg64 * nout);
+                    if (rc) return rc;
+                }
+                const dim3 tgrid((unsigned)((nx + 15) / 16), (unsigned)nx, (unsigned)((nz + 15) / 16));
+                hipLaunchKernelGGL(fp_transpose_kernel, tgrid, dim3(256), 0, ctx->stream, vol, g.img_dim,
+                                   g.img_dim_z, ctx->fp_pack);
+                hipLaunchKernelGGL(fp_ray64_kernel, dim3((unsigned)nb64), dim3(256), 0, ctx->stream, a,
+                                   plan->fp_ltab, ctx->fp_pack, (uint32_t)tbytes, (int)nwg64, (int)nseg64,
+                                   nxcd, ctx->fp_partial);
+                if (nseg64 > 1)
+                    hipLaunchKernelGGL(fp_segments_kernel, dim3(mct::blocks_for(nout, 256)), dim3(256), 0,
+                                       ctx->stream, ctx->fp_partial, nout, (int)nseg64, out);
+                MCT_LAUNCHED();
+                ctx->fp_last = MCT_FP_CELLS;
+                return MCT_OK;
+            }
+        }
         // 32-bit record offsets (and 24-bit row strides for the address
         // arithmetic); gridDim.y of the pack launch
         const bool fits = nc >= 1 && nz >= 1 && bytes < ((int64_t)1 << 32) - 64 &&
@@ -789,7 +990,6 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
             const bool two = !g.cone && tr <= 4;  // a second, y-major order
             rc = mct::ensure_buffer(ctx, &ctx->fp_pack, &ctx->fp_pack_cap, (two ? 2 : 1) * (bytes / 4));
             if (rc) return rc;
-            const int64_t nout = (int64_t)a.nrows * a.nviews * a.nu_out;
             if (nseg > 1) {
                 rc = mct::ensure_buffer(ctx, &ctx->fp_partial, &ctx->fp_partial_cap, nseg * nout);
                 if (rc) return rc;
