@@ -190,6 +190,7 @@ int mct_destroy(mct_ctx *ctx) {
     if (ctx->aux_dev) hipFree(ctx->aux_dev);
     if (ctx->fp_pack) hipFree(ctx->fp_pack);
     if (ctx->fp_partial) hipFree(ctx->fp_partial);
+    if (ctx->bp_rec) hipFree(ctx->bp_rec);
     for (mct::BatchSlot &b : ctx->batch_slot) {
         if (b.ev) hipEventSynchronize(b.ev);
         if (b.host) hipHostFree(b.host);

@@ -36,6 +36,9 @@ struct mct_ctx {
     int64_t fp_pack_cap = 0;  // floats
     float *fp_partial = nullptr;  // per-segment partial sums of split rays
     int64_t fp_partial_cap = 0;
+    // fan IR back-projector: the (masked) sinogram in slice-chunk records
+    float *bp_rec = nullptr;
+    int64_t bp_rec_cap = 0;
     // partial sums of the deterministic FP64 reductions
     double *partials = nullptr;
     int64_t partials_cap = 0;

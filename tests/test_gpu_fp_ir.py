@@ -122,13 +122,27 @@ def test_ray_mask_fan_is_ones(ctx):
     assert bool((mask == 1).all())
 
 
-@pytest.mark.parametrize("kind", ["fan", "cone", "cone_masked"])
+@pytest.mark.parametrize("kind", ["fan", "cone", "cone_masked", "fan_chunks", "fan_chunks_masked",
+                                  "fan_chunks_vh"])
 def test_backproject_ir(ctx, kind):
-    spec = fan_case(nx=48, nv=40, nu=70) if kind == "fan" else cone_case(nx=32, nz=16, nv=30)
+    """fan: 2 slices (one thread per column and view walks the sinogram
+    rows); fan_chunks*: 8 slices or more, read from the slice-chunk records
+    (11 and 13 slices: a partly filled last chunk; _vh: a voxel height, so a
+    weight per slice); the corners of the 48 x 0.6 mm image leave the
+    detector (truncated columns)."""
+    if kind == "fan":
+        spec = fan_case(nx=48, nv=40, nu=70)
+    elif kind.startswith("fan_chunks"):
+        spec = fan_case(nx=48, nv=40, nu=70, rows={"fan_chunks": 11, "fan_chunks_masked": 13,
+                                                   "fan_chunks_vh": 9}[kind], rot_deg=7.0, off=0.9)
+        if kind == "fan_chunks_vh":
+            spec.vox_h = 0.4
+    else:
+        spec = cone_case(nx=32, nz=16, nv=30)
     rng = np.random.default_rng(4)
     sgm = rng.standard_normal((spec.n_v, spec.n_views, spec.n_u)).astype(np.float32)
     mask = None
-    if kind == "cone_masked":
+    if kind.endswith("masked"):
         mask = (rng.uniform(0, 1, sgm.shape) > 0.3).astype(np.float32)
     plan = Plan(ctx, spec)
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device=DEV)
@@ -136,6 +150,8 @@ def test_backproject_ir(ctx, kind):
     got = host(vol)
     ref = recon.backproject_ir(sgm, oracle_dict(spec), ray_mask=mask, dtype=np.float32)
     assert int(((got != 0) & (ref == 0)).sum()) == 0
+    if kind.startswith("fan"):
+        assert int(((got == 0) & (ref != 0)).sum()) == 0 and 0 < int((ref == 0).sum()) < ref.size // 2
     assert rel_l2(got, ref) <= 1e-5
 
 

@@ -124,12 +124,13 @@ def _cgls_jobs(n, seed=21):
     return jobs
 
 
-@pytest.mark.parametrize("n", [3, 5])
+@pytest.mark.parametrize("n", [3, 5, 9, 16])
 def test_stacked_cgls_equals_single_runs(n):
     """n CGLS jobs run as one stacked problem give each job's own iterates
     bit for bit (slices independent, per-job scalars, the same ray split as
     one slice: with 5 slices the forward projector's row bands are padded,
-    and the split still follows the rays)."""
+    and the split still follows the rays; from 8 slices on the
+    back-projector reads slice-chunk records, 9: a partly filled chunk)."""
     jobs = _cgls_jobs(n)
     ctx = Context(0)
     dev = torch.device("cuda", 0)
