@@ -2,15 +2,24 @@
 1024 x 256 flat detector (FP32 compute), on one MI355X.
 
 One step = one pass of the FDK hot path over the resident sinogram:
-restore the raw sinogram (device copy), cosine weighting, ramp filter
-(FP32 direct or MFMA), repack into the back-projector's tap layout,
+cosine weighting, ramp filter (split-BF16 MFMA by default; the FP32 filter's
+step is timed beside it), repack into the back-projector's tap layout,
 pixel-driven back-projection.  value = GUPS = voxel-view updates of the step
-(512*512*256*1080) / step time.  The roofline object is the back-projection
-kernel's, timed with HIP events on the stream it runs on; its algorithmic
-bytes per update are 4 taps x 4 B (FP32 storage; 2 B for FP16/BF16) plus
-4 B per voxel written.  cpu_baseline: the oracle's C/OpenMP restatement of
-the same back-projection on a slab of slices (all views, all columns),
-timed on the host cores.
+(512*512*256*1080) / step time.
+
+roofline: the back-projection kernel's, timed with HIP events on the stream
+it runs on.  Its algorithmic bytes per launch are the compulsory ones: the
+filtered sinogram read once (4 B per detector sample, 2 B with FP16/BF16
+storage) and the volume written once (4 B per voxel), against HBM's 8 TB/s.
+The kernel is bound by VALU issue, not by HBM: the line carries the VALU
+issue fraction and the measured HBM traffic from the counter record of the
+same kernel sources (a JSON block in a profiles/*.md summary whose source
+hash matches; null otherwise), and the per-update gather figure as context.
+
+cpu_baseline: the oracle's C/OpenMP restatement of the same back-projection
+on a slab of slices (all views, all columns) on all host cores, plus a
+1-core run and the weighting, filter, forward-projection and IR
+back-projection restatements on samples of their workloads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f32]
 """
@@ -19,21 +28,19 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
+ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 # Device memory the timed steps' private input copies may take.
 RESIDENT_INPUT_BYTES = 32 << 30
-# HBM bytes per back-projection launch of this workload, measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a 1 GiB copy's counts
-# (FETCH x2, WRITE x1); profiles/fdk_c1_kernels.md.  Keyed by
-# (kernel, storage, --bp-tile, --bp-batch); other settings report null.
-MEASURED_TRAFFIC = {("direct", "f32", 0, 0): 94.2e9, ("lds", "f32", 0, 0): 77.6e9}
-# Per-CU fill rates on CDNA: vector L1 64 B/clk, LDS 128 B/clk; MI355X peak
-# engine clock 2.4 GHz.
+# Per-CU fill rates on CDNA: vector L1 64 B/clk, LDS 128 B/clk; one wave64
+# FP32 VALU instruction per CU per clock; MI355X peak engine clock 2.4 GHz.
 GATHER_BYTES_PER_CLK_PER_CU = {"direct": 64, "lds": 128}
 GATHER_PATH = {"direct": "vector L1", "lds": "LDS"}
 KERNEL_NAME = {"direct": "bp_cone_axial_kernel", "lds": "bp_cone_lds_kernel"}
@@ -43,7 +50,7 @@ CLOCK_GHZ = 2.4
 def gather_roofline(kernel: str, gather_bytes: float, ms: float) -> dict:
     """Every update reads its four taps (16 B FP32, 8 B FP16/BF16) through the
     vector L1 (direct kernel) or from LDS (staged kernel): that path's
-    ceiling."""
+    ceiling (context, not the bound)."""
     import torch
 
     if kernel not in GATHER_PATH:
@@ -53,6 +60,46 @@ def gather_roofline(kernel: str, gather_bytes: float, ms: float) -> dict:
     ach = gather_bytes / (ms * 1e-3) / 1e9
     return {"path": GATHER_PATH[kernel], "achieved": round(ach, 1), "peak": round(peak, 1),
             "unit": "GB/s", "frac": round(ach / peak, 4), "cus": cus}
+
+
+def counter_record(kernel: str, workload: str, source: str):
+    """The counter record of `kernel` on `workload` (a ```json block of a
+    profiles/*.md summary) whose source hash equals that of the sources at
+    hand; (record, where) or (None, why not)."""
+    from reconstruction_program_taichi_amd import _build
+
+    want = _build.kernel_source_hash(source)
+    why = f"no counter record of {kernel} on this workload"
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for blk in re.findall(r"```json\n(.*?)\n```", f.read_text(), re.S):
+            try:
+                rec = json.loads(blk)
+            except ValueError:
+                continue
+            if rec.get("counters_of") != kernel or rec.get("workload") != workload:
+                continue
+            if rec.get("source_hash") == want:
+                return rec, f"profiles/{f.name}"
+            why = f"profiles/{f.name} holds counters of other sources ({rec.get('source_hash')} != {want})"
+    return None, why
+
+
+def measured_bounds(rec: dict, where: str, ms: float, cus: int) -> dict:
+    """HBM traffic (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain
+    copy's readings) and VALU issue of one launch from the counter record,
+    over this run's launch time (ms) on cus compute units."""
+    c, cal = rec["per_dispatch"], rec["calibration"]
+    traffic = (c["FETCH_SIZE_KiB"] * 1024.0 * cal["fetch_factor"]
+               + c["WRITE_SIZE_KiB"] * 1024.0 * cal["write_factor"])
+    t = ms * 1e-3
+    out = {"profile": where, "source_hash": rec["source_hash"], "hbm_bytes": traffic,
+           "hbm_frac": round(traffic / t / (HBM_PEAK_GBS * 1e9), 4),
+           "valu_insts": c["SQ_INSTS_VALU"],
+           "valu_issue_frac": round(c["SQ_INSTS_VALU"] / (cus * CLOCK_GHZ * 1e9 * t), 4)}
+    if c.get("GRBM_GUI_ACTIVE"):
+        # issue over the cycles the GPU was busy with the dispatch (clock as run)
+        out["valu_issue_frac_busy_cycles"] = round(c["SQ_INSTS_VALU"] / (cus * c["GRBM_GUI_ACTIVE"]), 4)
+    return out
 
 
 def c1_config(storage: str, filter_method: str) -> dict:
@@ -70,26 +117,130 @@ def c1_config(storage: str, filter_method: str) -> dict:
     }
 
 
-def cpu_baseline(fbp, filt_host: np.ndarray, nslices: int) -> dict:
-    """Times the oracle's C back-projection on nslices central slices."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import cbp
+def counter_workload(args) -> str:
+    """The key of this run's back-projection in the counter records."""
+    return (f"cone FDK 512x512x256, 1080 views, 1024x256; {args.storage} taps; "
+            f"tile {args.bp_tile}, batch {args.bp_batch}")
+
+
+def fp32_filter_step(args, raw, vol, steps: int) -> dict:
+    """The same FDK step with the FP32 direct ramp filter (C1's arithmetic),
+    timed beside the headline's split-BF16 filter on the same inputs."""
+    import torch
+
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+
+    f = Mgfbp(c1_config(args.storage, "f32"))
+    f.ctx().set_bp_kernel(args.bp_kernel)
+    if args.bp_tile or args.bp_batch:
+        f.ctx().set_bp_tile(args.bp_tile, args.bp_batch)
+    out = torch.empty_like(vol)
+    work = raw.clone()
+    f.reconstruct(work, out)  # plan tables, taps
+    inputs = [raw.clone() for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        f.reconstruct(inputs[i], out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    del inputs, work, out
+    nx, nz, nv = vol.shape[2], vol.shape[0], raw.shape[1]
+    f.plan().close()
+    f.ctx().close()
+    return {"ms_per_step": round(ms, 3), "gups": round(float(nx) * nx * nz * nv / (ms * 1e-3) / 1e9, 2),
+            "steps": steps}
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(fbp, filt_host: np.ndarray, raw_rows: np.ndarray, vol_host: np.ndarray,
+                 nslices: int) -> dict:
+    """Reference-algorithm CPU restatements (the oracle's C/OpenMP code and
+    numpy), timed on samples of the workloads: the headline back-projection
+    on nslices central slices with every host thread and on 2 slices with
+    one, and the weighting, FP32 ramp filter, cone forward projection and fan
+    IR back-projection legs, each with its full-workload extrapolation."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import cbp, recon
+    from reconstruction_program_taichi_amd import geometry as geo
 
     spec = fbp.geometry_spec()
     g = dict(img_dim=spec.img_dim, img_dim_z=spec.img_dim_z, pix=spec.pix, vox_h=spec.vox_h,
              sid=spec.sid, sdd=spec.sdd, angles=spec.angles, u=spec.u, v=spec.v, det_h=spec.det_h,
              cone=True, cz=spec.cz, total_scan_angle=spec.total_scan_angle, short_scan=False)
-    z0 = spec.img_dim_z // 2 - nslices // 2
-    cbp.lib()  # build/load outside the timed region
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    out = cbp.backproject_fbp(filt_host, g, z0, z0 + nslices)
-    dt = time.perf_counter() - t0
-    upd = float(spec.img_dim) ** 2 * nslices * spec.n_views
-    return {"value": upd / dt / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
-            "sample": f"FP32 back-projection of {nslices} central slices x 512x512 columns x "
-                      f"1080 views ({upd:.3g} updates) in {dt:.2f} s; oracle/bp_oracle.c, OpenMP",
-            "_check": out}
+    nx, nz, nv, nrows, nu = spec.img_dim, spec.img_dim_z, spec.n_views, spec.n_v, spec.n_u
+    cbp.lib()  # build/load outside the timed regions
+    threads = cbp.max_threads()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t0
+
+    # back-projection, all threads (the headline CPU figure)
+    z0 = nz // 2 - nslices // 2
+    out, dt = timed(lambda: cbp.backproject_fbp(filt_host, g, z0, z0 + nslices))
+    upd = float(nx) ** 2 * nslices * nv
+    res = {"value": upd / dt / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+           "sample": f"FP32 back-projection of {nslices} central slices x 512x512 columns x "
+                     f"1080 views ({upd:.3g} updates) in {dt:.2f} s; oracle/bp_oracle.c, OpenMP "
+                     f"({threads} threads)",
+           "cpu_model": cpu_model(), "label": "reference-algorithm CPU restatement",
+           "_check": out}
+    legs = {}
+    # the same on one core
+    cbp.set_threads(1)
+    try:
+        _, dt1 = timed(lambda: cbp.backproject_fbp(filt_host, g, nz // 2, nz // 2 + 1))
+    finally:
+        cbp.set_threads(threads)
+    legs["backprojection_1core"] = {"gups": float(nx) ** 2 * nv / dt1 / 1e9, "cores": 1,
+                                    "sample": f"the central slice in {dt1:.2f} s"}
+    # weighting (numpy, one core), 16 detector rows
+    rows = raw_rows.shape[0]
+    _, dtw = timed(lambda: recon.weight_sinogram(raw_rows, spec.u, spec.v[:rows], spec.angles, spec.sdd,
+                                                 spec.total_scan_angle, False))
+    legs["weighting"] = {"samples_per_s": raw_rows.size / dtw, "cores": 1,
+                         "sample": f"{rows} detector rows x {nv} views x {nu} in {dtw:.2f} s (numpy)",
+                         "full_step_s": round(dtw * nrows / rows, 2)}
+    # FP32 ramp filter (O(n^2) per row, all threads), the same 16 detector rows
+    taps = fbp.recon_kernel()
+    du = float(abs(spec.u[1] - spec.u[0]))
+    _, dtf = timed(lambda: cbp.filter_rows(raw_rows, taps, du))
+    legs["ramp_filter"] = {"rows_per_s": rows * nv / dtf, "cores": threads,
+                           "sample": f"{rows * nv} rows of {nu} in {dtf:.2f} s (oracle C, OpenMP)",
+                           "full_step_s": round(dtf * nrows / rows, 2)}
+    legs["fdk_step_estimate_s"] = round(legs["weighting"]["full_step_s"] + legs["ramp_filter"]["full_step_s"]
+                                        + dt * nz / nslices, 1)
+    # cone forward projection of the FDK volume, 2 of the 1080 views
+    k = 2
+    gfp = dict(g, angles=spec.angles[:k], fpj_step=0.2)
+    _, dtp = timed(lambda: cbp.forward_project(vol_host, gfp))
+    legs["forward_projection"] = {"rays_per_s": k * nrows * nu / dtp, "cores": threads,
+                                  "sample": f"{k} views x {nrows} x {nu} rays, step 0.2 voxel, in {dtp:.2f} s "
+                                            "(oracle C, OpenMP)",
+                                  "full_1080_views_s": round(dtp * nv / k, 1)}
+    # fan IR back-projection, C3 geometry (512x512, 120 views x 800), 16 of the 64 slices
+    ang = geo.angle_array(120, 2 * geo.PI, 0.0)
+    gir = dict(img_dim=512, img_dim_z=16, pix=0.3, vox_h=0.0, sid=500.0, sdd=1000.0, angles=ang,
+               u=geo.det_pos_array(800, 800, -0.4, 0.0), v=geo.det_pos_array(16, 16, 0, 0),
+               cone=False, det_h=1.0)
+    sg = np.ones((16, 120, 800), np.float32)
+    _, dti = timed(lambda: cbp.backproject_ir(sg, gir))
+    legs["ir_backprojection"] = {"gups": 512.0 * 512 * 16 * 120 / dti / 1e9, "cores": threads,
+                                 "sample": f"16 slices x 512x512 x 120 views in {dti:.2f} s (oracle C, OpenMP)",
+                                 "full_64_slices_s": round(dti * 4, 2)}
+    res["legs"] = legs
+    return res
 
 
 def main() -> int:
@@ -101,10 +252,15 @@ def main() -> int:
     # bf16x3: split-BF16 Toeplitz GEMM on MFMA, 2.5e-6 rel-L2 vs FP64 (FP32-grade)
     ap.add_argument("--filter", default="bf16x3", choices=["f32", "bf16", "bf16x3"])
     ap.add_argument("--cpu-slices", type=int, default=32)
+    ap.add_argument("--fp32-filter-steps", type=int, default=3,
+                    help="timed FDK steps with the FP32 ramp filter beside the headline (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=36,
                     help="jobs in the served-stream comparison (0: skip it)")
     ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the timed steps (no FP32-filter, PCIe-inclusive or streamed legs): "
+                         "every back-projection launch is then a whole-scan one (profiler runs)")
     ap.add_argument("--bp-tile", type=int, default=0, help="slices per thread (0: library default)")
     ap.add_argument("--bp-batch", type=int, default=0, help="loads per batch (0: library default)")
     ap.add_argument("--bp-kernel", default="auto", choices=["auto", "direct", "lds"],
@@ -212,9 +368,14 @@ def main() -> int:
     U = float(nx) * nx * nz * nv
     gups = U / (ms_step * 1e-3) / 1e9
     tap_bytes = 4 if args.storage == "f32" else 2
-    bytes_alg = U * 4 * tap_bytes + float(nx) * nx * nz * 4
+    # compulsory bytes of one back-projection: the filtered sinogram once, the volume once
+    bytes_alg = float(spec.n_v) * nv * spec.n_u * tap_bytes + float(nx) * nx * nz * 4
     achieved = bytes_alg / (bp_ms * 1e-3) / 1e9
     kernel = fbp.ctx().bp_last_kernel()
+    kname = KERNEL_NAME.get(kernel, kernel)
+    rec, where = counter_record(kname, counter_workload(args), "mct_bp.hip")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    measured = measured_bounds(rec, where, bp_ms, cus) if rec else {"profile": None, "why": where}
     line = {
         "metric": "FDK/backprojection GUPS", "value": round(gups, 2), "unit": "GUPS",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -230,20 +391,24 @@ def main() -> int:
                       if fbp.tuned else {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((kernel, args.storage, args.bp_tile,
-                                                      args.bp_batch)),
-                     "kernel": KERNEL_NAME.get(kernel, kernel),
-                     "model": f"gather bytes: 4 taps x {tap_bytes} B per update + 4 B per voxel; "
-                              "the taps are served by LDS/L1/L2, so frac > 1 is expected",
+                     "traffic": measured.get("hbm_bytes"), "kernel": kname,
+                     "model": (f"compulsory bytes: the filtered sinogram once ({tap_bytes} B per "
+                               "detector sample) + the volume once (4 B per voxel)"),
+                     "binding": {"resource": "VALU issue", "frac": measured.get("valu_issue_frac"),
+                                 "frac_busy_cycles": measured.get("valu_issue_frac_busy_cycles")},
+                     "measured": measured,
                      "gather": gather_roofline(kernel, U * 4 * tap_bytes, bp_ms)},
     }
+    if args.filter != "f32" and args.fp32_filter_steps > 0 and not args.no_extras:
+        line["config"]["fp32_filter_step"] = fp32_filter_step(args, raw, vol, args.fp32_filter_steps)
     raw_h = torch.empty(raw.shape, dtype=raw.dtype, pin_memory=True)
     raw_h.copy_(raw)
-    line["config"]["pcie_inclusive"] = pcie_inclusive(raw_h, work, vol, fbp)
-    line["config"]["streamed"] = streamed_inclusive(raw_h, vol, args.storage, args.filter)
+    if not args.no_extras:
+        line["config"]["pcie_inclusive"] = pcie_inclusive(raw_h, work, vol, fbp)
+        line["config"]["streamed"] = streamed_inclusive(raw_h, vol, args.storage, args.filter)
     if not args.no_cpu:
         filt_host = fbp.last_filtered.cpu().numpy()
-        cb = cpu_baseline(fbp, filt_host, args.cpu_slices)
+        cb = cpu_baseline(fbp, filt_host, raw[:16].cpu().numpy(), vol.cpu().numpy(), args.cpu_slices)
         check = cb.pop("_check")
         z0 = nz // 2 - args.cpu_slices // 2
         gpu_slab = vol[z0:z0 + args.cpu_slices].cpu().numpy()

@@ -77,3 +77,30 @@ int bp_oracle_fbp(const float *filt, int nrows, int nviews, int nu, const float 
     }
     return 0;
 }
+
+/*
+ * Horizontal pass of Mgfbp.ConvolveSgmAndKernel (run_mgfbp.py:1033-1046),
+ * FP32: out[r][k] = (sum over m ascending of x[r][m] * taps[k + n-1 - m]) *
+ * scale, O(n^2) per row as in the Taichi kernel, parallel over the rows.
+ * x, out: [rows][n]; taps: 2n - 1 entries.
+ */
+int filter_oracle_f32(const float *x, int64_t rows, int n, const float *taps, float scale,
+                      float *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < rows; ++r) {
+        const float *xr = x + r * n;
+        float *o = out + r * n;
+        for (int k = 0; k < n; ++k) {
+            float acc = 0.0f;
+            for (int m = 0; m < n; ++m) acc += xr[m] * taps[k + n - 1 - m];
+            o[k] = acc * scale;
+        }
+    }
+    return 0;
+}
+
+/* OpenMP thread count of the restatements (the CPU baseline's 1-core and
+ * all-core legs). */
+#include <omp.h>
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }

@@ -31,7 +31,34 @@ def lib():
             fn.argtypes = ([pt, C.c_int, C.c_int, C.c_int, pt, pt, ct, ct, pt, pt, C.c_int, C.c_int]
                            + [ct] * 5 + [pt, ct, ct, ct, C.c_int, pt])
             fn.restype = C.c_int
+        _lib.filter_oracle_f32.argtypes = [fp, C.c_int64, C.c_int, fp, C.c_float, fp]
+        _lib.filter_oracle_f32.restype = C.c_int
+        _lib.oracle_set_threads.argtypes = [C.c_int]
+        _lib.oracle_set_threads.restype = None
+        _lib.oracle_max_threads.argtypes = []
+        _lib.oracle_max_threads.restype = C.c_int
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the C restatements from now on."""
+    lib().oracle_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())
+
+
+def filter_rows(x, taps, scale):
+    """FP32 ramp filter of every row of x (last axis), as recon.filter_rows
+    with dtype float32."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    n = f.shape[-1]
+    t = np.ascontiguousarray(taps, dtype=np.float32)
+    assert t.size == 2 * n - 1
+    out = np.empty_like(f)
+    lib().filter_oracle_f32(_p(f), f.size // n, n, _p(t), float(scale), _p(out))
+    return out
 
 
 def _p(a):

@@ -28,6 +28,21 @@ CFLAGS = [
 ]
 
 
+def kernel_source_hash(source: str) -> str:
+    """Digest of what one csrc file's kernels are compiled from: the compile
+    flags, that file, every csrc header and include/mct.h.  A counter
+    profile records it, so a reader can tell whether the profile belongs to
+    the build at hand."""
+    import hashlib
+
+    h = hashlib.sha256(" ".join(CFLAGS).encode())
+    files = [CSRC / source, *sorted(CSRC.glob("*.h")), PKG.parent / "include" / "mct.h"]
+    for f in files:
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():

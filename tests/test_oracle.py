@@ -746,3 +746,20 @@ def test_ir_solve_with_c_projectors_equals_numpy_projectors():
     a = recon.ir_solve(sgm, g, x0, 1e-2, 1e-4, 1, 3)
     b = recon.ir_solve(sgm, g, x0, 1e-2, 1e-4, 1, 3, projectors=(cbp.forward_project, cbp.backproject_ir))
     assert rel_l2(b, a) < 1e-11
+
+
+def test_c_filter_equals_numpy_oracle_bit_for_bit():
+    """The C restatement of the FP32 ramp filter (CPU baseline leg) sums in
+    the numpy oracle's order: identical FP32 results, on 1 and on all
+    threads."""
+    rng = np.random.default_rng(70)
+    x = rng.standard_normal((3, 5, 64)).astype(np.float32)
+    taps = recon.ramlak_taps(64, 0.4)
+    want = recon.filter_rows(x, taps, 0.4, np.float32)
+    n0 = cbp.max_threads()
+    try:
+        for t in (1, n0):
+            cbp.set_threads(t)
+            assert np.array_equal(cbp.filter_rows(x, taps, 0.4), want)
+    finally:
+        cbp.set_threads(n0)
