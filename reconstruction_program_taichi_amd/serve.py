@@ -228,7 +228,9 @@ def cgls_stack_key(job: Job) -> tuple:
 def run_plain(jobs: list[Job], device: int = 0) -> dict:
     """One job at a time, synchronously, in arrival order: fresh plan,
     allocation, taps upload and host <-> device copies paid per job (the
-    reference's per-call pattern).  Returns latency/throughput statistics."""
+    reference's per-call pattern); sinograms come from and results go to
+    pinned host memory, as in the server.  Returns latency/throughput
+    statistics."""
     ctx = Context(device)
     dev = torch.device("cuda", device)
     phases = {"setup": 0.0, "h2d": 0.0, "compute": 0.0, "d2h": 0.0}
@@ -258,7 +260,10 @@ def run_plain(jobs: list[Job], device: int = 0) -> dict:
             x = torch.empty(job.vol_shape(), dtype=torch.float32, device=dev)
             plan.backproject_fbp(packed, job.storage, layout, x)
         t = mark("compute", t)
-        job.result = x.cpu()
+        # into pinned host memory, as the server's results
+        host = torch.empty(job.vol_shape(), dtype=torch.float32, pin_memory=True)
+        host.copy_(x)
+        job.result = host
         mark("d2h", t)
         plan.close()
         job.t_done = time.perf_counter() - t0
