@@ -450,6 +450,20 @@ def fp_scalars(g, count_mode=0, dtype=np.float32):
     return dict(lmin=lmin, diag=diag, count=count, x0=x0, y0=y0, zbase=zbase)
 
 
+def helical_z_per_view(ang, v, n_v_full, pitch, sdd, sid, dtype=np.float32):
+    """Gantry travel per view of a helical scan (run_mgfpj.py:518-523): the
+    scan angle estimated from the first and last view angles and the view
+    count, over 2 pi, gives the laps; pitch x (detector height at the
+    isocentre: |v[1] - v[0]| x n_v_full x sid / sdd) per lap."""
+    T = _t(dtype)
+    ang = np.asarray(ang, dtype=dtype)
+    v = np.asarray(v, dtype=dtype)
+    nv = ang.shape[0]
+    tsa = np.abs(ang[nv - 1] - ang[0]) / T(nv - 1) * T(nv)
+    laps = tsa / (T(PI) * T(2))
+    return T(T(pitch) * (laps / T(nv)) * (np.abs(v[1] - v[0]) * T(n_v_full)) / (T(sdd) / T(sid)))
+
+
 def _pm_rays(g, ang, Ui, R, J, dtype):
     """P-matrix branch of the forward projector (run_mgfpj_v2.py:334-355,
     run_mgfpj_v3.py:286-307, run_mgfbp_iterative_recon_IRN_CGLS.py:598-622):

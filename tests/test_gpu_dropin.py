@@ -54,6 +54,39 @@ def test_run_mgfbp_file_roundtrip(tmp_path):
     assert rel_l2(out, ref_hu) <= 1e-5
 
 
+def test_run_mgfbp_post_log_images_input(tmp_path):
+    """InputFileForm post_log_images (run_mgfbp.py:1206-1222): the file holds
+    one [rows][n_u] image per view; the recon-range rows of the first Views
+    images are read (offset / gap per image) and back-projected.  The result
+    equals the same data given as a sinogram file whose first slice is the
+    detector's top row (post-log images are upright, run_mgfbp.py:177-181)
+    bit for bit, and the oracle chain on the rows it should have read."""
+    from reconstruction_program_taichi_amd.fbp import Mgfbp, run_mgfbp
+
+    (tmp_path / "in").mkdir()
+    (tmp_path / "in2").mkdir()
+    cone = dict(ConeBeam=True, DetectorElementHeight=0.6, ImageDimensionZ=8, VoxelHeight=0.5,
+                DetectorElementCountVertical=20, DetectorElementVerticalReconRange=[4, 15],
+                DetectorOffsetVertical=0.0, SinogramHeight=26, WaterMu=0.02)
+    rng = np.random.default_rng(7)
+    images = rng.uniform(0, 1, (26, 20, 72)).astype(np.float32)  # [view][row][col]
+    images.tofile(tmp_path / "in" / "sgm_p.raw")
+    cfg = _fan_cfg(tmp_path, views=24, InputFileForm="post_log_images", **cone)
+    out = run_mgfbp(_write_cfg(tmp_path / "c.jsonc", cfg))
+    # the same rows as a sinogram file [row][view][col]
+    np.ascontiguousarray(images.transpose(1, 0, 2)).tofile(tmp_path / "in2" / "sgm_p.raw")
+    cfg2 = _fan_cfg(tmp_path, views=24, FirstSinogramSliceIsDetectorTopRow=True, **cone)
+    cfg2.update(InputDir=str(tmp_path / "in2"), OutputDir=str(tmp_path / "out2"))
+    out2 = run_mgfbp(_write_cfg(tmp_path / "c2.jsonc", cfg2))
+    assert out.shape == (8, 40, 40) and np.array_equal(out, out2)
+    spec = Mgfbp(cfg).geometry_spec()
+    sgm = np.ascontiguousarray(images[:24, 4:16, :].transpose(1, 0, 2))
+    w = recon.weight_sinogram(sgm, spec.u, spec.v, spec.angles, spec.sdd, spec.total_scan_angle, False)
+    fr = recon.filter_rows(w, geo.hamming_taps(72, 0.8, 1.0, 500.0), 0.8, np.float32)
+    ref = (recon.backproject_fbp(fr, oracle_dict(spec), np.float32) / 0.02 - 1) * 1000
+    assert rel_l2(out, ref) <= 1e-5
+
+
 def test_run_mgfpj_cone_file_roundtrip(tmp_path):
     from reconstruction_program_taichi_amd.fpj import Mgfpj, run_mgfpj
 
@@ -153,7 +186,11 @@ def test_ir_cone_solver_matches_oracle_loop(tmp_path):
 
 
 def test_run_mgfbp_ir_end_to_end(tmp_path):
-    from reconstruction_program_taichi_amd.ir import run_mgfbp_ir
+    """run_mgfbp_ir (run_mgfbp_iterative_recon_IRN_CGLS.py:14-49, 138-176)
+    against the oracle chain: FBP seed in HU, HU -> mu, IRN-CGLS-TV from that
+    seed on the same sinogram, mu -> HU; compared in mu units."""
+    from reconstruction_program_taichi_amd.fbp import Mgfbp
+    from reconstruction_program_taichi_amd.ir import Mgfbp_ir, run_mgfbp_ir
 
     (tmp_path / "in").mkdir()
     cfg = _ir_cfg(tmp_path, WaterMu=0.02)
@@ -164,3 +201,17 @@ def test_run_mgfbp_ir_end_to_end(tmp_path):
     assert out.shape == (3, 40, 40) and np.all(np.isfinite(out))
     written = np.fromfile(tmp_path / "out" / "p_rec_c.raw", dtype=np.float32).reshape(out.shape)
     assert np.array_equal(written, out)
+    # oracle chain
+    spec = Mgfbp(cfg).geometry_spec()
+    w = recon.weight_sinogram(sgm, spec.u, spec.v, spec.angles, spec.sdd, spec.total_scan_angle, False)
+    fr = recon.filter_rows(w, geo.hamming_taps(72, 0.8, 1.0, 500.0), 0.8, np.float32)
+    seed_hu = ((recon.backproject_fbp(fr, oracle_dict(spec), np.float32) / np.float32(0.02) - 1)
+               * 1000).astype(np.float32)
+    seed_mu = ((seed_hu / np.float32(1000) + 1) * np.float32(0.02)).astype(np.float32)
+    ir = Mgfbp_ir(cfg)
+    x = recon.ir_solve(sgm, oracle_dict(ir.ir_geometry_spec()), seed_mu, ir.coef_lambda, ir.beta_tv,
+                       1, 4)
+    got_mu = (out.astype(np.float64) / 1000 + 1) * 0.02
+    err = rel_l2(got_mu, x)
+    print(f"run_mgfbp_ir vs oracle chain (FBP seed, HU round trip, IRN-CGLS): rel-L2 {err:.2e} (mu)")
+    assert err <= 1e-4

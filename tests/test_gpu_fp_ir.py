@@ -36,20 +36,39 @@ def _vol(spec, seed):
 
 
 def _fp_spec(kind):
+    """fan_rows70 / fan_curved: 70 slices, so the fan path that traces one ray
+    per wavefront over 64 slices runs (a full and a partial slice tile);
+    cone_helical: gantry travel per view from HelicalPitch."""
     os_ = 2 if kind == "fan_os2" else 1
-    if kind.startswith("fan"):
+    if kind in ("fan_rows70", "fan_curved"):
+        spec = fan_case(nx=40, nv=24, nu=60, rows=70, rot_deg=11.0, off=0.7)
+        spec.curved = kind == "fan_curved"
+    elif kind.startswith("fan"):
         spec = fan_case(nx=40, nv=24, nu=60 * os_, w=0.8 / os_, rows=3)
     elif kind == "cone_offcentre":
         spec = cone_case(nx=33, nz=15, nv=20, nu=48, nrows=24)
         spec.cx, spec.cy = 1.3, -0.8
     else:
         spec = cone_case(nx=32, nz=16, nv=20, nu=48, nrows=24, top_row=(kind == "cone_top"))
+        if kind == "cone_curved":
+            spec.curved = True
+        if kind == "cone_helical":
+            spec.helical, spec.helical_pitch = True, 0.8
     spec.fpj_step = 0.2
     return spec, os_
 
 
+def _fp_oracle_dict(spec):
+    g = oracle_dict(spec)
+    if spec.helical:
+        g["zpv"] = recon.helical_z_per_view(spec.angles, spec.v, spec.n_v_full, spec.helical_pitch,
+                                            spec.sdd, spec.sid)
+    return g
+
+
 @pytest.mark.parametrize("kernel", ["cells", "direct"])
-@pytest.mark.parametrize("kind", ["fan", "cone", "cone_top", "fan_os2", "cone_offcentre"])
+@pytest.mark.parametrize("kind", ["fan", "cone", "cone_top", "fan_os2", "cone_offcentre", "fan_rows70",
+                                  "fan_curved", "cone_curved", "cone_helical"])
 def test_forward_project(ctx, kind, kernel):
     spec, os_ = _fp_spec(kind)
     vol = _vol(spec, 1)
@@ -62,15 +81,16 @@ def test_forward_project(ctx, kind, kernel):
     finally:
         ctx.set_fp_kernel("auto")
     got = host(out)
-    ref = recon.forward_project(vol, oracle_dict(spec), oversample=os_, dtype=np.float32)
+    ref = recon.forward_project(vol, _fp_oracle_dict(spec), oversample=os_, dtype=np.float32)
     assert rel_l2(got, ref) <= 1e-5
 
 
-@pytest.mark.parametrize("kind", ["fan", "cone", "cone_offcentre", "fan_os2"])
+@pytest.mark.parametrize("kind", ["fan", "cone", "cone_offcentre", "fan_os2", "fan_rows70"])
 def test_forward_project_cells_equal_direct(ctx, kind):
     """Same samples, in-bounds tests and summation order; the cells hold
-    FP64-rounded bilinear coefficients where the direct kernel interpolates
-    raw voxels in FP32: equal to rounding, and zero on the same rays."""
+    FP32 bilinear coefficients {a, b-a, c-a, (d-c)-(b-a)} where the direct
+    kernel interpolates raw voxels: equal to rounding, and zero on the same
+    rays."""
     spec, os_ = _fp_spec(kind)
     vol = dev(_vol(spec, 4))
     plan = Plan(ctx, spec)

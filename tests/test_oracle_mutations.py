@@ -56,6 +56,18 @@ MUTATIONS = [
     ("cone forward projection without the step length",
      "val = ((T(1) - zw) * lo + zw * hi) * step * diag", "val = ((T(1) - zw) * lo + zw * hi) * step",
      [P.pin_cone_fp_is_trilinear_line_sum, P.pin_cone_fp_of_ball_is_chord]),
+    ("helical travel from n_v_full - 1 rows",
+     "(np.abs(v[1] - v[0]) * T(n_v_full))", "(np.abs(v[1] - v[0]) * T(n_v_full - 1))",
+     [P.pin_helical_travel_is_pitch_times_isocentre_coverage_per_lap]),
+    ("helical travel without the sid / sdd demagnification",
+     "/ (T(sdd) / T(sid)))", "* (T(sdd) / T(sid)))",
+     [P.pin_helical_travel_is_pitch_times_isocentre_coverage_per_lap]),
+    ("helical forward projection without the gantry travel",
+     "zs = zpv * J.astype(dtype)", "zs = zpv * J.astype(dtype) * T(0)",
+     [P.pin_helical_fp_view_equals_circular_fp_of_the_shifted_volume]),
+    ("helical rays aimed from the unmoved source height",
+     "ex, ey, ez = dx - sid, dy - T(0), dz - zs", "ex, ey, ez = dx - sid, dy - T(0), dz",
+     [P.pin_helical_fp_view_equals_circular_fp_of_the_shifted_volume]),
     ("fan back-projection with the view weight squared",
      "yield ok, (sid * w) * val * db[j] * div", "yield ok, (sid * w) * val * db[j] * db[j] * div",
      [P.pin_fbp_of_fp_reproduces_phantom]),

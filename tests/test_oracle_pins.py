@@ -444,3 +444,44 @@ def test_dbt_taps_add_a_triangle_response():
     resp = np.real(np.sum(add[None, :] * np.exp(-2j * np.pi * f[:, None] * k[None, :]), axis=1)) * w
     want = kt * np.clip(1 - f / kt, 0, None)
     assert np.allclose(resp, want, atol=0.01 * kt)
+
+
+def pin_helical_travel_is_pitch_times_isocentre_coverage_per_lap(R):
+    """Over one lap of views the helical z travel is pitch x the detector's
+    height at the isocentre (n rows x row pitch x sid / sdd); two laps of
+    views at the same spacing move the same per view."""
+    for nv_lap, pitch, rows, dv, sid, sdd in [(360, 1.0, 16, 0.5, 500.0, 1000.0),
+                                              (90, 0.7, 64, 0.25, 600.0, 1100.0)]:
+        for laps in (1, 2):
+            nv = nv_lap * laps
+            ang = np.arange(nv, dtype=np.float64) * (2 * np.pi / nv_lap)
+            v = (np.arange(rows) - (rows - 1) / 2) * dv
+            zpv = float(R.helical_z_per_view(ang, v, rows, pitch, sdd, sid, dtype=np.float64))
+            want = pitch * rows * dv * sid / sdd
+            assert abs(zpv * nv_lap - want) <= 1e-9 * want
+
+
+def pin_helical_fp_view_equals_circular_fp_of_the_shifted_volume(R):
+    """View j of a helical cone forward projection (source and detector moved
+    by j zpv in z) equals view j of the circular projection of the volume
+    moved by -j zpv (the volume centre cz lowered by j zpv)."""
+    spec = cone_case(nx=14, nz=10, nv=6, nu=20, nrows=12)
+    spec.fpj_step = 0.5
+    g = oracle_dict(spec)
+    rng = np.random.default_rng(81)
+    vol = rng.uniform(0, 1, (10, 14, 14))
+    zpv = 0.37
+    hel = R.forward_project(vol, dict(g, zpv=zpv), dtype=np.float64)
+    for j in (0, 2, 5):
+        gj = dict(g, angles=np.asarray(g["angles"])[j:j + 1], cz=g["cz"] - j * zpv)
+        circ = R.forward_project(vol, gj, dtype=np.float64)[:, 0]
+        assert np.allclose(hel[:, j], circ, rtol=1e-9, atol=1e-9 * np.abs(circ).max())
+    assert np.abs(hel[:, 5] - R.forward_project(vol, g, dtype=np.float64)[:, 5]).max() > 1e-3
+
+
+def test_helical_travel_is_pitch_times_isocentre_coverage_per_lap():
+    pin_helical_travel_is_pitch_times_isocentre_coverage_per_lap(recon)
+
+
+def test_helical_fp_view_equals_circular_fp_of_the_shifted_volume():
+    pin_helical_fp_view_equals_circular_fp_of_the_shifted_volume(recon)
