@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 7
+#define MCT_ABI_VERSION 8
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -301,7 +301,7 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
  *                  detector rows, so a sample is one or two coalesced 16-byte
  *                  loads (MCT_ERR_UNSUPPORTED where it does not fit).
  * Both take the same samples with the same in-bounds tests; the cells hold
- * FP64-rounded coefficients, so the sums agree to rounding. */
+ * FP32 bilinear coefficients, so the sums agree to rounding. */
 typedef enum mct_fp_kernel { MCT_FP_AUTO = 0, MCT_FP_DIRECT = 1, MCT_FP_CELLS = 2 } mct_fp_kernel;
 int mct_set_fp_kernel(mct_ctx *ctx, int32_t kind);
 /* Kernel the last mct_forward_project call on ctx ran (MCT_FP_DIRECT or
@@ -313,6 +313,16 @@ int mct_fp_last_kernel(const mct_ctx *ctx);
  * The split changes the order of summation: results with different settings
  * agree to rounding, results with the same setting are identical. */
 int mct_set_fp_split(mct_ctx *ctx, int32_t waves_per_cu);
+
+/* No counterpart in the reference: for callers that capture the library's
+ * launches in a HIP graph.  A captured graph holds the addresses of the
+ * context's scratch (filter planes and tap panels, the forward projector's
+ * packed volume and partial sums, the IR back-projector's records, reduction
+ * partials) and the contents of its cached tables (ramp taps, vertical
+ * kernel, helical positions).  With frozen = 1, any call that would grow,
+ * reallocate or rewrite one of them fails with MCT_ERR_INVALID instead
+ * (calls of the captured sizes and taps still run); 0 lifts it. */
+int mct_set_frozen(mct_ctx *ctx, int32_t frozen);
 
 /* Mgfbp_ir.GenEffectiveMapForwardProjectionAgent
  * run_mgfbp_iterative_recon_IRN_CGLS.py:679-833: 1 for rays whose samples

@@ -24,7 +24,7 @@ MCT_FILTER_F32_DIRECT, MCT_FILTER_BF16_MFMA, MCT_FILTER_BF16X3_MFMA = 0, 1, 2
 MCT_BP_AUTO, MCT_BP_DIRECT, MCT_BP_LDS = 0, 1, 2
 MCT_FP_AUTO, MCT_FP_DIRECT, MCT_FP_CELLS = 0, 1, 2
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 
 class MctError(RuntimeError):
@@ -101,6 +101,7 @@ SIGNATURES = {
     "mct_set_fp_kernel": (C.c_int, [_P, _I32]),
     "mct_fp_last_kernel": (C.c_int, [_P]),
     "mct_set_fp_split": (C.c_int, [_P, _I32]),
+    "mct_set_frozen": (C.c_int, [_P, _I32]),
     "mct_backproject_batch": (C.c_int, [_P, C.POINTER(BpJob), _I32]),
     "mct_bp_batch_create": (C.c_int, [_P, C.POINTER(BpJob), _I32, C.POINTER(_P)]),
     "mct_bp_batch_launch": (C.c_int, [_P]),

@@ -1558,6 +1558,9 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     MCT_CHECK(njobs >= 0 && njobs <= kMaxBatchJobs, "njobs must be in [0, %d] (got %d)", kMaxBatchJobs,
               njobs);
     if (njobs == 0) return MCT_OK;
+    // every call rewrites a staging slot that a captured copy would read
+    MCT_CHECK(!ctx->frozen, "context is frozen: mct_backproject_batch rewrites its staging slots; "
+                            "captured graphs use mct_bp_batch_create / mct_bp_batch_launch");
     const int dtype = jobs[0].dtype;
     const size_t args_bytes = sizeof(BpArgs) * (size_t)njobs;
     const size_t bytes = args_bytes + sizeof(int) * ((size_t)njobs + 1);

@@ -90,8 +90,15 @@ int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
+int refuse_frozen(const char *what, int64_t have, int64_t need) {
+    return set_error(MCT_ERR_INVALID,
+                     "context is frozen (a captured graph may use its scratch): %s would change "
+                     "(%lld -> %lld)", what, (long long)have, (long long)need);
+}
+
 int ensure_partials(mct_ctx *ctx, int64_t n) {
     if (ctx->partials_cap >= n) return MCT_OK;
+    if (ctx->frozen) return refuse_frozen("reduction partials", ctx->partials_cap, n);
     if (ctx->partials) {
         MCT_HIP(hipStreamSynchronize(ctx->stream));
         MCT_HIP(hipFree(ctx->partials));
@@ -106,6 +113,7 @@ int ensure_partials(mct_ctx *ctx, int64_t n) {
 
 int ensure_buffer(mct_ctx *ctx, float **buf, int64_t *cap, int64_t n) {
     if (*cap >= n) return MCT_OK;
+    if (ctx->frozen) return refuse_frozen("a scratch buffer (floats)", *cap, n);
     if (*buf) {
         MCT_HIP(hipStreamSynchronize(ctx->stream));
         MCT_HIP(hipFree(*buf));
@@ -122,6 +130,7 @@ int upload_cached(mct_ctx *ctx, float **buf, int64_t *cap, uint64_t *hash, const
                   int64_t n) {
     const uint64_t h = fnv1a(host, sizeof(float) * (size_t)n, (uint64_t)n);
     if (*buf && *cap >= n && *hash == h) return MCT_OK;
+    if (ctx->frozen) return refuse_frozen("a cached device table (floats)", *cap, n);
     int rc = ensure_buffer(ctx, buf, cap, n);
     if (rc) return rc;
     *hash = 0;  // invalid until the copy is queued
@@ -264,6 +273,12 @@ int mct_set_fp_split(mct_ctx *ctx, int32_t waves_per_cu) {
     if (waves_per_cu < 0 || waves_per_cu > 4096)
         return mct::set_error(MCT_ERR_INVALID, "waves_per_cu must be in [0, 4096] (got %d)", waves_per_cu);
     ctx->fp_split = waves_per_cu;
+    return MCT_OK;
+}
+
+int mct_set_frozen(mct_ctx *ctx, int32_t frozen) {
+    if (!ctx) return mct::set_error(MCT_ERR_INVALID, "ctx is NULL");
+    ctx->frozen = frozen ? 1 : 0;
     return MCT_OK;
 }
 

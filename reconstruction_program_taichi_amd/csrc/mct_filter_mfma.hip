@@ -268,6 +268,7 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
     // set (cached in the context)
     const uint64_t h = fnv1a(taps, sizeof(float) * (size_t)(2 * n - 1), (uint64_t)n);
     if (!ctx->toep || ctx->toep_n != n || ctx->toep_hash != h) {
+        if (ctx->frozen) return mct::refuse_frozen("the ramp tap panel (n)", ctx->toep_n, n);
         std::vector<uint16_t> hl(2 * (size_t)ntaps);
         double g = 0.0;
         for (int i = 0; i < ntaps; ++i) {
@@ -305,6 +306,7 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
     const int64_t plane = n_rows * (int64_t)npad;
     const int64_t bytes = plane * 2 * (split ? 2 : 1);
     if (ctx->fsplit_bytes < bytes) {
+        if (ctx->frozen) return mct::refuse_frozen("the filter input planes (bytes)", ctx->fsplit_bytes, bytes);
         if (ctx->fsplit) {
             MCT_HIP(hipStreamSynchronize(ctx->stream));
             MCT_HIP(hipFree(ctx->fsplit));

@@ -31,6 +31,7 @@ struct mct_ctx {
     int fp_kernel = 0; // forward projector: 0 automatic, 1 direct taps, 2 packed cells
     int fp_last = 0;   // kernel of the last mct_forward_project (mct_fp_last_kernel)
     int fp_split = 0;  // cell projector: target wavefronts per CU when splitting rays (0: default)
+    int frozen = 0;    // 1: scratch and cached tables may not grow or change (mct_set_frozen)
     // packed volume of the forward projector (two orders of bilinear cells)
     float *fp_pack = nullptr;
     int64_t fp_pack_cap = 0;  // floats
@@ -122,6 +123,9 @@ constexpr int kBufferDword3 = 0x00020000;
 
 // Records the message returned by mct_last_error() for the calling thread.
 int set_error(int code, const char *fmt, ...);
+// Error for a frozen context's scratch or cached table that would change
+// from `have` to `need` (mct_set_frozen).
+int refuse_frozen(const char *what, int64_t have, int64_t need);
 
 #define MCT_HIP(call)                                                           \
     do {                                                                        \

@@ -113,6 +113,12 @@ class Context:
         launch has fewer than waves_per_cu wavefronts per CU (0: default)."""
         call("mct_set_fp_split", self.h, int(waves_per_cu))
 
+    def freeze(self, frozen: bool = True) -> None:
+        """While frozen, calls that would grow, reallocate or rewrite the
+        context's scratch or cached tables (which captured graphs address)
+        raise MctError instead; calls of the captured sizes and taps run."""
+        call("mct_set_frozen", self.h, int(bool(frozen)))
+
     def fp_last_kernel(self) -> str:
         k = _abi.lib().mct_fp_last_kernel(self.h)
         return {0: "none", 1: "direct", 2: "cells"}.get(k, str(k))

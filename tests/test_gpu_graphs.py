@@ -109,3 +109,68 @@ def _oracle(spec):
     from tests._cases import oracle_dict
 
     return oracle_dict(spec)
+
+
+@pytest.mark.parametrize("method", ["bf16x3", "f32"])
+def test_frozen_context_keeps_a_captured_pipeline_valid(method):
+    """While the context is frozen, calls that would grow its scratch or
+    rewrite its cached taps raise MctError (the captured graph addresses
+    them); same-size calls run, and the graph keeps replaying the eager
+    result.  Unfrozen, the larger calls run again."""
+    from reconstruction_program_taichi_amd._abi import MctError
+    from reconstruction_program_taichi_amd.device import Context
+
+    ctx = Context(0)
+    spec = cone_case(nu=80, nrows=32, nz=30)
+    rng = np.random.default_rng(9)
+    raw = torch.from_numpy(rng.uniform(0, 1, (spec.n_v, spec.n_views, spec.n_u)).astype(np.float32)).cuda()
+    taps = recon.ramlak_taps(spec.n_u, 0.6, np.float32)
+    plan = Plan(ctx, spec)
+    work, filt = torch.empty_like(raw), torch.empty_like(raw)
+    packed = torch.empty(ctx.pack(raw, "f32", "quad").numel(), dtype=torch.uint8, device="cuda")
+    vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
+
+    def pipeline():
+        plan.weight(work)
+        ctx.filter_rows(work, filt, taps, 0.6, method)
+        ctx.pack_into(filt, "f32", "quad", packed)
+        plan.backproject_fbp(packed, "f32", "quad", vol)
+
+    work.copy_(raw)
+    pipeline()
+    torch.cuda.synchronize()
+    want = vol.clone()
+    g = _capture(ctx, pipeline)
+    big = torch.rand((4 * spec.n_v, spec.n_views, spec.n_u), device="cuda")
+    big_out = torch.empty_like(big)
+    fp_out = torch.empty((spec.n_v, spec.n_views, spec.n_u), device="cuda")
+    ctx.freeze()
+    try:
+        work.copy_(raw)
+        pipeline()  # the captured sizes and taps: runs
+        torch.cuda.synchronize()
+        assert torch.equal(vol, want)
+        with pytest.raises(MctError, match="frozen"):
+            ctx.filter_rows(work, filt, taps * 2, 0.6, method)  # other taps
+        if method != "f32":
+            with pytest.raises(MctError, match="frozen"):
+                ctx.filter_rows(big, big_out, taps, 0.6, method)  # larger input planes
+        with pytest.raises(MctError, match="frozen"):
+            plan.forward_project(vol, fp_out)  # no forward-projector scratch yet
+        for rep in range(3):
+            vol.fill_(float("nan"))
+            work.copy_(raw)
+            junk = torch.randn(1 << 16, device="cuda").sum()
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert float(junk) == float(junk)
+            assert torch.equal(vol, want), f"replay {rep}"
+    finally:
+        ctx.freeze(False)
+    del g
+    ctx.filter_rows(big, big_out, taps * 2, 0.6, method)
+    plan.forward_project(vol, fp_out)
+    torch.cuda.synchronize()
+    plan.close()
+    ctx.close()
