@@ -269,7 +269,7 @@ def main() -> int:
                     help="time the back-projector candidates during warm-up and keep the fastest")
     ap.add_argument("--sinogram", default="phantom", choices=["phantom", "random"],
                     help="analytic ellipsoid projections, or seeded uniform noise (profiler runs)")
-    ap.add_argument("--workload", default="fdk", choices=["fdk", "fbp2d", "fp3d", "ir", "filter", "helical", "pmatrix"],
+    ap.add_argument("--workload", default="fdk", choices=["fdk", "fbp2d", "fp3d", "fpfan", "ir", "filter", "helical", "pmatrix"],
                     help="fdk: the bench line; the others print their own measurement")
     ap.add_argument("--fp-views", type=int, default=1080, help="fp3d: views projected")
     ap.add_argument("--fp-kernel", default="auto", choices=["auto", "direct", "cells"],

@@ -558,6 +558,29 @@ __device__ __forceinline__ float4 corner_coef(float a, float b, float c, float d
     return cell_coef(a, b, c, d);
 }
 
+// Adds the samples of steps [s, e) of the current chunk, all in cell q, in
+// step order (weights from LDS).
+__device__ __forceinline__ float walk_steps(const float2 *__restrict__ wts, int s, int e, float4 q,
+                                           float sum) {
+    for (; s + 1 < e; s += 2) {
+        const float2 w0 = wts[s], w1 = wts[s + 1];
+        sum += cell_lerp(q, w0.x, w0.y);
+        sum += cell_lerp(q, w1.x, w1.y);
+    }
+    if (s < e) {
+        const float2 w = wts[s];
+        sum += cell_lerp(q, w.x, w.y);
+    }
+    return sum;
+}
+
+// End of the run of in-volume steps starting at step s0 (< 64), capped at end.
+__device__ __forceinline__ int run_end(uint64_t m_in, int s0, int end) {
+    const uint64_t out = ~m_in >> s0;
+    const int r = out ? s0 + (int)__builtin_ctzll(out) : 64;
+    return r < end ? r : end;
+}
+
 __device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
                              __amdgpu_buffer_rsrc_t rsrc, uint32_t slice4, int ui, int j, int seg,
                              int nseg, float2 *__restrict__ wts) {
@@ -610,10 +633,7 @@ __device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
         c_in = (m_in >> 63) & 1;
         // steps at the start that continue the previous chunk's cell
         const int first = m_new ? __builtin_ctzll(m_new) : 64;
-        for (int s = 0; s < first && ((m_in >> s) & 1); ++s) {
-            const float2 w = wts[s];
-            sum += cell_lerp(cur, w.x, w.y);
-        }
+        sum = walk_steps(wts, 0, run_end(m_in, 0, first), cur, sum);
         while (m_new) {
             int n[4];
             float v[4][4];
@@ -642,10 +662,7 @@ __device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
                 if (t < cnt) {
                     const float4 q = corner_coef(v[t][0], v[t][1], v[t][2], v[t][3]);
                     const int end = t + 1 < cnt ? n[t + 1] : tail;
-                    for (int s = n[t]; s < end && ((m_in >> s) & 1); ++s) {
-                        const float2 w = wts[s];
-                        sum += cell_lerp(q, w.x, w.y);
-                    }
+                    sum = walk_steps(wts, n[t], run_end(m_in, n[t], end), q, sum);
                     cur = q;
                 }
             }

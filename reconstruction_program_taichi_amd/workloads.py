@@ -6,6 +6,8 @@
          30 seeded random ellipsoids) into 1080 views x 256 rows x 1024
          columns at 0.4 mm, SID 750 / SDD 1250 mm, step 0.2 of the voxel
          diagonal (optionally a leading subset of the views).
+  fpfan: the IR loop's fan forward projection alone: 64 slices of 512 x 512
+         into 120 views x 800 columns (fbp2d's geometry), ray mask applied.
   ir:    IRN-CGLS-TV on 64 fan slices of 512 x 512 (geometry of fbp2d with
          120 views), 5 reweighting passes x 20 CG iterations, Lambda 1e-3,
          BetaTV 1e-6, slices coupled through TV as the reference does.
@@ -111,6 +113,28 @@ def fp3d(args) -> dict:
             "kernel": fpj.ctx().fp_last_kernel(),
             "ms": round(t * 1e3, 3), "rays": rays, "steps_per_ray": steps,
             "samples_per_s": round(S / t / 1e9, 2), "unit": "G samples/s (reference step count)"}
+
+
+def fpfan(args) -> dict:
+    """The IR loop's forward projection alone: 64 fan slices of 512 x 512
+    (0.3 mm) -> 120 views x 800 columns (0.4 mm), SID 500 / SDD 1000, the
+    ray mask applied, step 0.2 voxel."""
+    from .ir import Mgfbp_ir
+
+    dev = torch.device("cuda", 0)
+    irr = Mgfbp_ir({**_fan_cfg(120, 64), "Lambda": 1e-3})
+    spec = irr.ir_geometry_spec()
+    plan = Plan(irr.ctx(), spec)
+    x = torch.rand((64, 512, 512), generator=torch.Generator(device=dev).manual_seed(1), device=dev)
+    out = torch.empty((64, 120, 800), dtype=torch.float32, device=dev)
+    mask = torch.empty_like(out)
+    plan.ray_mask(mask)
+    irr.ctx().set_fp_kernel(getattr(args, "fp_kernel", "auto"))
+    plan.forward_project(x, out, mask, 1)  # warm-up
+    t = _sync_time(lambda: plan.forward_project(x, out, mask, 1), max(1, args.steps))
+    return {"workload": "fan forward projection 64 x 512x512 -> 120 views x 800 (IR loop geometry)",
+            "kernel": irr.ctx().fp_last_kernel(), "ms": round(t * 1e3, 3),
+            "rays": 64 * 120 * 800, "checksum": float(out.double().sum())}
 
 
 def ir(args) -> dict:
@@ -259,5 +283,5 @@ def pmatrix(args) -> dict:
 
 
 def run(name: str, args) -> dict:
-    return {"fbp2d": fbp2d, "fp3d": fp3d, "ir": ir, "filter": filter_rows, "helical": helical,
-            "pmatrix": pmatrix}[name](args)
+    return {"fbp2d": fbp2d, "fp3d": fp3d, "fpfan": fpfan, "ir": ir, "filter": filter_rows,
+            "helical": helical, "pmatrix": pmatrix}[name](args)
