@@ -545,45 +545,25 @@ __global__ __launch_bounds__(256) void fp_rec_kernel(FpArgs a, const float *__re
 // column is the same line (the rows differ only in the slice they sample),
 // so a wavefront traces one ray in 64 slices (lane = slice) and computes the
 // ray's sample positions, weights and in-bounds tests once, 64 steps at a
-// time with a lane per step.  It then walks the steps in order: for each cell
-// the ray enters, the four corner voxels of the 64 slices (256 B each, from
-// the slice-fastest copy of the volume, which the last-level cache holds
-// across views) are loaded once (four cells' loads in flight together) and
-// turned into the cell's bilinear coefficients as the records are; each
-// step's weights go through LDS (written by the step's lane, read by all
-// lanes; one wavefront's LDS operations complete in order).  Same samples, tests,
-// coefficients and order of summation as the record kernel (steps outside
-// the volume add nothing).
+// time with a lane per step.  The steps a ray spends in one cell sample the
+// same bilinear patch, so they add n a + Sx (b-a) + Sy (c-a) + Sxy ((d-c) -
+// (b-a)) with n the step count and Sx, Sy, Sxy the sums of their weights
+// wx, wy, wx wy: the first lane of each such run forms the sums (runs are a
+// few steps long) and puts them in LDS, where every lane reads them
+// (broadcast; one wavefront's LDS operations complete in order).  For each
+// cell the ray enters, the four corner voxels of the 64 slices (256 B each,
+// from the slice-fastest copy of the volume, which the last-level cache holds
+// across views) are loaded once, four cells' loads in flight together, and
+// turned into the cell's bilinear coefficients as the records are.  Same
+// samples, in-bounds tests and coefficients as the record kernel; the
+// per-cell grouping of the sum changes its rounding only.
 __device__ __forceinline__ float4 corner_coef(float a, float b, float c, float d) {
     return cell_coef(a, b, c, d);
 }
 
-// Adds the samples of steps [s, e) of the current chunk, all in cell q, in
-// step order (weights from LDS).
-__device__ __forceinline__ float walk_steps(const float2 *__restrict__ wts, int s, int e, float4 q,
-                                           float sum) {
-    for (; s + 1 < e; s += 2) {
-        const float2 w0 = wts[s], w1 = wts[s + 1];
-        sum += cell_lerp(q, w0.x, w0.y);
-        sum += cell_lerp(q, w1.x, w1.y);
-    }
-    if (s < e) {
-        const float2 w = wts[s];
-        sum += cell_lerp(q, w.x, w.y);
-    }
-    return sum;
-}
-
-// End of the run of in-volume steps starting at step s0 (< 64), capped at end.
-__device__ __forceinline__ int run_end(uint64_t m_in, int s0, int end) {
-    const uint64_t out = ~m_in >> s0;
-    const int r = out ? s0 + (int)__builtin_ctzll(out) : 64;
-    return r < end ? r : end;
-}
-
 __device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
                              __amdgpu_buffer_rsrc_t rsrc, uint32_t slice4, int ui, int j, int seg,
-                             int nseg, float2 *__restrict__ wts) {
+                             int nseg, float2 *__restrict__ wts, float4 *__restrict__ sums) {
     const Ray r = make_ray(a, ui, 0, j);
     int k_lo, k_hi;
     clip_range(a, r, 0.f, false, k_lo, k_hi);
@@ -615,7 +595,7 @@ __device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
         const float fy = (yr - a.y0) * (-a.inv_pix);
         const int xi = floor_to_int(fx), yi = floor_to_int(fy);
         const bool in = k < k_hi && (uint32_t)xi < nc && (uint32_t)yi < nc;
-        // this chunk's weights, read back by every lane (broadcast) step by step
+        // this chunk's weights, read by the first lane of each run
         __builtin_amdgcn_wave_barrier();
         wts[lane] = make_float2(__builtin_amdgcn_fractf(fx), __builtin_amdgcn_fractf(fy));
         __builtin_amdgcn_wave_barrier();
@@ -628,44 +608,68 @@ __device__ float trace_ray64(const FpArgs &a, const float *__restrict__ ltab,
         }
         const bool fresh = in && !(p_in && p_off == off);
         const uint64_t m_in = __builtin_amdgcn_ballot_w64(in);
-        uint64_t m_new = __builtin_amdgcn_ballot_w64(fresh);
+        const uint64_t m_new = __builtin_amdgcn_ballot_w64(fresh);
         c_off = (uint32_t)__builtin_amdgcn_readlane((int)off, 63);
         c_in = (m_in >> 63) & 1;
-        // steps at the start that continue the previous chunk's cell
-        const int first = m_new ? __builtin_ctzll(m_new) : 64;
-        sum = walk_steps(wts, 0, run_end(m_in, 0, first), cur, sum);
-        while (m_new) {
+        // Each run of steps in one cell (a cell start, or step 0 continuing the
+        // previous chunk's cell, up to the next cell start or step outside)
+        // adds n a + Sx (b-a) + Sy (c-a) + Sxy ((d-c)-(b-a)) with the sums of
+        // its steps' weights: the run's first lane forms them.
+        const uint64_t m_stop = m_new | ~m_in;
+        float sx = 0.f, sy = 0.f, sxy = 0.f;
+        int e = lane;
+        if (fresh || (lane == 0 && in)) {
+            const uint64_t rest = lane < 63 ? m_stop >> (lane + 1) : 0;
+            e = rest ? lane + 1 + (int)__builtin_ctzll(rest) : 64;
+            for (int s2 = lane; s2 < e; ++s2) {
+                const float2 w = wts[s2];
+                sx += w.x;
+                sy += w.y;
+                sxy = fmaf(w.x, w.y, sxy);
+            }
+        }
+        // the run sums of every lane, read back by all lanes (broadcast)
+        __builtin_amdgcn_wave_barrier();
+        sums[lane] = make_float4((float)(e - lane), sx, sy, sxy);
+        __builtin_amdgcn_wave_barrier();
+        if (!((m_new & 1) || !(m_in & 1))) {  // step 0 continues the previous chunk's cell
+            const float4 c = sums[0];
+            sum += fmaf(c.w, cur.w, fmaf(c.z, cur.z, fmaf(c.y, cur.y, c.x * cur.x)));
+        }
+        uint64_t m_rem = m_new;
+        while (m_rem) {
             int n[4];
             float v[4][4];
             int cnt = 0;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                n[t] = 64;
-                if (m_new) {
-                    n[t] = __builtin_ctzll(m_new);
-                    m_new &= m_new - 1;
+                n[t] = 0;
+                if (m_rem) {
+                    n[t] = __builtin_ctzll(m_rem);
+                    m_rem &= m_rem - 1;
                     ++cnt;
                 }
             }
+            float4 c[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
                 if (t < cnt) {
-                    const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)off, n[t]) + slice4;
-                    v[t][0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o, 0, 0));
-                    v[t][1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o + xb, 0, 0));
-                    v[t][2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o + yb, 0, 0));
-                    v[t][3] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o + yb + xb, 0, 0));
+                    // the cell's byte offset is uniform: the scalar offset of the loads
+                    const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)off, n[t]);
+                    const uint32_t o4 = o + slice4;
+                    v[t][0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o4, 0, 0));
+                    v[t][1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o4 + xb, 0, 0));
+                    v[t][2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o4 + yb, 0, 0));
+                    v[t][3] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, o4 + yb + xb, 0, 0));
+                    c[t] = sums[n[t]];
                 }
-            const int tail = m_new ? __builtin_ctzll(m_new) : 64;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 4; ++t)
                 if (t < cnt) {
                     const float4 q = corner_coef(v[t][0], v[t][1], v[t][2], v[t][3]);
-                    const int end = t + 1 < cnt ? n[t + 1] : tail;
-                    sum = walk_steps(wts, n[t], run_end(m_in, n[t], end), q, sum);
+                    sum += fmaf(c[t].w, q.w, fmaf(c[t].z, q.z, fmaf(c[t].y, q.y, c[t].x * q.x)));
                     cur = q;
                 }
-            }
         }
     }
     return sum * dl;
@@ -692,10 +696,11 @@ __global__ __launch_bounds__(256) void fp_ray64_kernel(FpArgs a, const float *__
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)volt, 0, (int)bytes, kBufferDword3);
     const uint32_t slice4 = (uint32_t)(inside ? vi : 0) * 4u;
-    __shared__ float2 wts[4][64];  // per wavefront: the current chunk's step weights
+    __shared__ float2 wts[4][64];   // per wavefront: the current chunk's step weights
+    __shared__ float4 sums[4][64];  // per wavefront: the current chunk's run sums
     float acc = 0.f;
     for (int i = 0; i < a.os; ++i)
-        acc += trace_ray64(a, ltab, rsrc, slice4, uo * a.os + i, j, seg, nseg, wts[wave]);
+        acc += trace_ray64(a, ltab, rsrc, slice4, uo * a.os + i, j, seg, nseg, wts[wave], sums[wave]);
     if (!inside) return;
     const float v = masked ? 0.f : acc / (float)a.os;
     if (nseg == 1)

@@ -87,10 +87,10 @@ def test_forward_project(ctx, kind, kernel):
 
 @pytest.mark.parametrize("kind", ["fan", "cone", "cone_offcentre", "fan_os2", "fan_rows70"])
 def test_forward_project_cells_equal_direct(ctx, kind):
-    """Same samples, in-bounds tests and summation order; the cells hold
-    FP32 bilinear coefficients {a, b-a, c-a, (d-c)-(b-a)} where the direct
-    kernel interpolates raw voxels: equal to rounding, and zero on the same
-    rays."""
+    """Same samples and in-bounds tests; the cells hold FP32 bilinear
+    coefficients {a, b-a, c-a, (d-c)-(b-a)} where the direct kernel
+    interpolates raw voxels, and the 64-slice fan path (fan_rows70) adds each
+    cell's steps as one term: equal to rounding, and zero on the same rays."""
     spec, os_ = _fp_spec(kind)
     vol = dev(_vol(spec, 4))
     plan = Plan(ctx, spec)

@@ -207,6 +207,49 @@ def test_c3_irn_cgls_tv_matches_oracle_loop():
     assert np.abs(ref).max() > 1e-3 and e <= 1e-4
 
 
+def test_c3_full_size_projectors_match_oracle():
+    """The projector pair of every CG iteration at C3's full size (64 slices
+    of 512^2, 120 views x 800): the forward projection (64 slices: one ray
+    per wavefront over the slices, each cell's steps summed as one term) on a
+    subset of views, and the IR back-projection (slice-chunk records) of all
+    views, against the C restatement of the oracle."""
+    ir, spec, _ = _c3_inputs(n_slices=64)
+    plan = ir.ir_plan()
+    g = oracle_dict(spec)
+    views = [0, 37, 71, 119]
+    gv = dict(g, angles=spec.angles[views])
+    fp = torch.empty((64, 120, 800), dtype=torch.float32, device="cuda")
+    # a phantom (30 seeded ellipsoids, voxelised): against the FP32 oracle
+    ell = phantom.random_ellipsoids(30, seed=12, r_fov=R_FOV_C0, z_half=8.0)
+    x = phantom.voxelize(ell, 512, 64, 0.3, 0.3, clip_r=R_FOV_C0, supersample=1)
+    plan.forward_project(x, fp)
+    got = fp[:, views].cpu().numpy()
+    ref = cbp.forward_project(x.cpu().numpy(), gv, dtype=np.float32)
+    e = rel_l2(got, ref)
+    _say(f"C3 forward projection ({ir.ctx().fp_last_kernel()}), 64 slices x 4 views x 800: rel-L2 {e:.2e}")
+    assert ref.max() > 1.0 and e <= 1e-5 and np.array_equal(got == 0, ref == 0)
+    # white noise: FP32 formulations differ by ~2e-5 here, so against FP64:
+    # the kernel is no further from it than the FP32 oracle is
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.rand((64, 512, 512), generator=gen, device="cuda") * 0.02
+    plan.forward_project(x, fp)
+    got = fp[:, views].cpu().numpy()
+    r32 = cbp.forward_project(x.cpu().numpy(), gv, dtype=np.float32)
+    r64 = cbp.forward_project(x.cpu().numpy().astype(np.float64), gv, dtype=np.float64)
+    e_got, e_o32 = rel_l2(got, r64), rel_l2(r32, r64)
+    _say(f"C3 forward projection of white noise: rel-L2 vs FP64 {e_got:.2e} (FP32 oracle {e_o32:.2e})")
+    assert e_got <= 1.1 * e_o32
+    sgm = torch.rand((64, 120, 800), generator=gen, device="cuda")
+    bp = torch.empty((64, 512, 512), dtype=torch.float32, device="cuda")
+    plan.backproject_ir(sgm, None, bp)
+    got = bp.cpu().numpy()
+    ref = cbp.backproject_ir(sgm.cpu().numpy(), g, dtype=np.float32)
+    e = rel_l2(got, ref)
+    _say(f"C3 IR back-projection, 64 slices x 120 views: rel-L2 {e:.2e}, truncated voxels "
+         f"{int((ref == 0).sum())}")
+    assert e <= 1e-5 and np.array_equal(got == 0, ref == 0) and (ref == 0).any()
+
+
 def test_c3_piccs_matches_oracle_loop(tmp_path):
     from reconstruction_program_taichi_amd.ir import Mgfbp_ir_piccs
 
