@@ -7,6 +7,7 @@ current stream, so they order with torch's own work on that stream.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 import torch
@@ -63,6 +64,8 @@ class Context:
         h = C.c_void_p()
         call("mct_create", self.device, C.byref(h))
         self.h = h
+        self.bp_kernel = "auto"  # as last set with set_bp_kernel / set_fp_kernel
+        self.fp_kernel = "auto"
         self.bind_stream()
 
     def bind_stream(self, stream: torch.cuda.Stream | None = None) -> None:
@@ -98,6 +101,7 @@ class Context:
         """Cone-beam back-projector: 'auto', 'direct' (per-update gathers) or
         'lds' (detector windows staged in LDS)."""
         call("mct_set_bp_kernel", self.h, self.BP_KERNELS[kind])
+        self.bp_kernel = kind
 
     FP_KERNELS = {"auto": _abi.MCT_FP_AUTO, "direct": _abi.MCT_FP_DIRECT,
                   "cells": _abi.MCT_FP_CELLS}
@@ -107,6 +111,7 @@ class Context:
         as given) or 'cells' (bilinear cells in two orders, in context
         scratch)."""
         call("mct_set_fp_kernel", self.h, self.FP_KERNELS[kind])
+        self.fp_kernel = kind
 
     def set_fp_split(self, waves_per_cu: int = 0) -> None:
         """Cell projector: split each ray's steps over several wavefronts when a
@@ -290,6 +295,7 @@ class Plan:
     def __init__(self, ctx: Context, spec: GeometrySpec):
         self.ctx = ctx
         self.spec = spec
+        self._fallback_warned: set = set()
         st = spec.to_struct()
         h = C.c_void_p()
         call("mct_plan_create", ctx.h, C.byref(st), C.byref(h))
@@ -305,6 +311,32 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+    # ---- kernel fallbacks the caller should know about ------------------------------
+    def _warn_once(self, key: str, msg: str) -> None:
+        if key not in getattr(self, "_fallback_warned", set()):
+            self._fallback_warned = getattr(self, "_fallback_warned", set()) | {key}
+            warnings.warn(msg, RuntimeWarning, stacklevel=3)
+
+    def _report_fallback_bp(self, layout: str) -> None:
+        """The automatic choice runs the LDS-staged cone back-projector unless
+        its plan tables do not fit (window table over 2 GiB, or a detector
+        window over the LDS budget): then the direct kernel runs, about 1.8x
+        slower.  Said once per plan."""
+        s = self.spec
+        if (s.cone and layout == "quad" and self.ctx.bp_kernel == "auto"
+                and self.ctx.bp_last_kernel() == "direct"):
+            self._warn_once("bp", "cone back-projection ran the direct-gather kernel: the LDS-staged "
+                                  "kernel's tables do not fit this geometry (window table over 2 GiB "
+                                  "or detector window over the LDS budget); expect about 1.8x the time")
+
+    def _report_fallback_fp(self) -> None:
+        """The automatic choice runs the record forward projector unless its
+        records do not fit 32-bit offsets (about 4 GiB): then the direct
+        kernel runs, several times slower.  Said once per plan."""
+        if self.ctx.fp_kernel == "auto" and self.ctx.fp_last_kernel() == "direct":
+            self._warn_once("fp", "forward projection ran the direct kernel: the volume's records do not "
+                                  "fit 32-bit offsets (about 4 GiB); expect several times the time")
 
     # ---- shape checks against the plan's geometry (before any device call) --------
     def _check_sgm(self, t: torch.Tensor, name: str = "sinogram", views: int | None = None,
@@ -344,6 +376,7 @@ class Plan:
         self._check_packed(filt, dtype, layout, self.spec.n_views)
         self._check_vol(vol)
         call("mct_backproject_fbp", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout], _f32(vol))
+        self._report_fallback_bp(layout)
         return vol
 
     def weight_views(self, sgm: torch.Tensor, view_begin: int, bh_coef=None) -> torch.Tensor:
@@ -390,6 +423,7 @@ class Plan:
         self._check_vol(vol)
         call("mct_backproject_views", self.h, _ptr(filt), DTYPES[dtype], LAYOUTS[layout],
              int(view_begin), int(view_count), int(bool(finish)), _f32(vol))
+        self._report_fallback_bp(layout)
         return vol
 
     def forward_project(self, vol: torch.Tensor, out: torch.Tensor, ray_mask=None,
@@ -403,6 +437,7 @@ class Plan:
             self._check_sgm(ray_mask, "ray_mask", nu=s.n_u // oversample)
         call("mct_forward_project", self.h, _f32(vol), _f32(out),
              None if ray_mask is None else _f32(ray_mask), int(oversample))
+        self._report_fallback_fp()
         return out
 
     def ray_mask(self, out: torch.Tensor) -> torch.Tensor:

@@ -222,3 +222,41 @@ def test_ramp_filter_difference_form_is_exact(n):
     y = np.array([sum(D[m] * G[kk + n - m] for m in range(n + 1)) for kk in k])
     want = np.convolve(x, taps)[n - 1:2 * n - 1]
     assert np.allclose(y, want, rtol=1e-10, atol=1e-9 * np.abs(want).max())
+
+
+def test_kernel_fallbacks_are_reported_once_per_plan():
+    """When the automatic kernel choice has to fall back to a direct kernel
+    (tables or records too large for the geometry), the plan says so once,
+    as a RuntimeWarning; not when the caller asked for the direct kernel."""
+    import types
+    import warnings
+
+    from reconstruction_program_taichi_amd.device import Plan
+    from tests._cases import cone_case, fan_case
+
+    def plan_for(spec, bp_kernel="auto", fp_kernel="auto", last="direct"):
+        p = Plan.__new__(Plan)
+        p.spec, p.h = spec, None
+        p.ctx = types.SimpleNamespace(bp_kernel=bp_kernel, fp_kernel=fp_kernel,
+                                      bp_last_kernel=lambda: last, fp_last_kernel=lambda: last)
+        return p
+
+    cone = cone_case(nx=16, nz=8, nv=12, nu=20, nrows=10)
+    p = plan_for(cone)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        p._report_fallback_bp("quad")
+        p._report_fallback_bp("quad")
+        p._report_fallback_fp()
+        p._report_fallback_fp()
+    msgs = [str(x.message) for x in w if issubclass(x.category, RuntimeWarning)]
+    assert len(msgs) == 2 and "direct-gather" in msgs[0] and "records" in msgs[1]
+    quiet = [plan_for(cone, bp_kernel="direct", fp_kernel="direct"), plan_for(cone, last="lds"),
+             plan_for(fan_case(nx=16, nv=8, nu=20))]
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        for q in quiet[:2]:
+            q._report_fallback_bp("quad")
+        quiet[0]._report_fallback_fp()
+        quiet[2]._report_fallback_bp("pair")
+    assert not [x for x in w if issubclass(x.category, RuntimeWarning)]
