@@ -934,6 +934,16 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
     a.mask = ray_mask;
     a.out = out;
     mct_ctx *ctx = plan->ctx;
+    if (g.img_dim < 2) {
+        // a one-pixel image has no cell: no sample passes the in-bounds test
+        // 0 <= index, index + 1 < dim (run_mgfpj.py:572), so every line sum is 0
+        const int64_t nout = (int64_t)a.nrows * a.nviews * a.nu_out;
+        hipLaunchKernelGGL(fill_kernel, dim3(mct::blocks_for(nout, 256)), dim3(256), 0, ctx->stream, out,
+                           nout, 0.f);
+        MCT_LAUNCHED();
+        ctx->fp_last = ctx->fp_kernel == MCT_FP_DIRECT ? MCT_FP_DIRECT : MCT_FP_CELLS;
+        return MCT_OK;
+    }
     dim3 grid((a.nu_out + 63) / 64, (a.nrows + 3) / 4, a.nviews);
     if (ctx->fp_kernel != MCT_FP_DIRECT) {
         const int64_t nx = g.img_dim, nz = g.img_dim_z;
