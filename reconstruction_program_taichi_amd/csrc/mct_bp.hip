@@ -1181,8 +1181,10 @@ BpArgs make_args(const mct_plan *p) {
 #define MCT_AXIAL_LAUNCH(KERNEL, ZC, DT) \
     hipLaunchKernelGGL((KERNEL<ZC, DT>), dim3(nblocks), dim3(256), 0, ctx->stream, a)
 
-// Window tables larger than this are not built (the direct kernel runs).
-constexpr size_t kMaxWinTableBytes = size_t(2) << 30;
+// Window tables larger than this are not built (the direct kernel runs):
+// 16 GiB, e.g. a 1024^3 volume from 7600 views (8 B per tile, 16-slice
+// chunk and view; 2.3 GB at 1024^3 from 1080 views).
+constexpr size_t kMaxWinTableBytes = size_t(16) << 30;
 // Largest LDS window (cells) the staged kernel takes: 8 staging slots per
 // thread; two FP32 buffers of 2048 cells fill 64 KiB.
 constexpr int kLdsMaxCells = 2048;

@@ -320,14 +320,14 @@ class Plan:
 
     def _report_fallback_bp(self, layout: str) -> None:
         """The automatic choice runs the LDS-staged cone back-projector unless
-        its plan tables do not fit (window table over 2 GiB, or a detector
+        its plan tables do not fit (window table over 16 GiB, or a detector
         window over the LDS budget): then the direct kernel runs, about 1.8x
         slower.  Said once per plan."""
         s = self.spec
         if (s.cone and layout == "quad" and self.ctx.bp_kernel == "auto"
                 and self.ctx.bp_last_kernel() == "direct"):
             self._warn_once("bp", "cone back-projection ran the direct-gather kernel: the LDS-staged "
-                                  "kernel's tables do not fit this geometry (window table over 2 GiB "
+                                  "kernel's tables do not fit this geometry (window table over 16 GiB "
                                   "or detector window over the LDS budget); expect about 1.8x the time")
 
     def _report_fallback_fp(self) -> None:
