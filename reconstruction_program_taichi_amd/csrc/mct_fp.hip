@@ -968,10 +968,11 @@ int mct_forward_project(mct_plan *plan, const float *vol, float *out, const floa
         // in a second pass.  (512^2 fan slice, 240 views: 0.46 ms with 32
         // wavefronts per CU, 0.35 ms with 256.)  The split counts the rays
         // (wavefronts of 64 rays), so a problem of n rows with the target
-        // scaled by n splits each ray exactly as a one-row problem does; the
-        // parts come from each ray's own step range.
+        // scaled by n splits each ray exactly as a one-row problem does (the
+        // count is rows x the wavefronts of one row, so no rounding couples
+        // the rows); the parts come from each ray's own step range.
         const int64_t target = (int64_t)ctx->cu_count * (ctx->fp_split ? ctx->fp_split : 256);
-        const int64_t nwaves_rays = ((int64_t)a.nu_out * a.nrows * a.nviews + 63) / 64;
+        const int64_t nwaves_rays = ((int64_t)a.nu_out * a.nviews + 63) / 64 * a.nrows;
         int64_t nseg = (target + nwaves_rays - 1) / nwaves_rays;
         nseg = nseg < 1 ? 1 : (nseg > 16 ? 16 : nseg);
         const int nxcd = ctx->xcd_count > 0 ? ctx->xcd_count : 1;

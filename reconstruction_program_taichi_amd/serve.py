@@ -298,8 +298,10 @@ class Server:
         self._next_b = 0
         # CGLS jobs that are loaded wait per stream until it is idle, then go
         # as stacks of same-geometry jobs (one forward and back-projection
-        # per iteration for the whole stack)
-        self.max_cgls_stack = max(1, int(max_cgls_stack))
+        # per iteration for the whole stack).  At most 63 jobs: a fan problem
+        # of 64 or more slices takes the one-ray-per-wavefront projector,
+        # whose sums are grouped differently from a single slice's.
+        self.max_cgls_stack = min(63, max(1, int(max_cgls_stack)))
         self._cgls_wait: list[list[Job]] = [[] for _ in self.s_b]
         self._cgls_busy: list = [None for _ in self.s_b]
         self.cgls_stacks: list[int] = []  # size of every CGLS run launched
