@@ -39,9 +39,11 @@ ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 # Device memory the timed steps' private input copies may take.
 RESIDENT_INPUT_BYTES = 32 << 30
-# Per-CU fill rates on CDNA: vector L1 64 B/clk, LDS 128 B/clk; one wave64
+# Per-CU fill rates: vector L1 64 B/clk; LDS 256 B/clk for 128-bit reads on
+# CDNA4 (twice CDNA3's 128 B/clk: a read-only stream of ds_read_b128 measured
+# 131 TB/s on MI355X, above the 78.6 TB/s that 128 B/clk allows); one wave64
 # FP32 VALU instruction per CU per clock; MI355X peak engine clock 2.4 GHz.
-GATHER_BYTES_PER_CLK_PER_CU = {"direct": 64, "lds": 128}
+GATHER_BYTES_PER_CLK_PER_CU = {"direct": 64, "lds": 256}
 GATHER_PATH = {"direct": "vector L1", "lds": "LDS"}
 KERNEL_NAME = {"direct": "bp_cone_axial_kernel", "lds": "bp_cone_lds_kernel"}
 CLOCK_GHZ = 2.4

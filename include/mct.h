@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 8
+#define MCT_ABI_VERSION 9
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -39,8 +39,9 @@ typedef enum mct_dtype { MCT_F32 = 0, MCT_F16 = 1, MCT_BF16 = 2 } mct_dtype;
 
 /* memory layout of a filtered sinogram read by the back-projector
  *   PLAIN: [rows][views][n_u] scalars (the reference's field order)
- *   QUAD : [views][rows-1][n_u-1] x 4 taps {(r,c),(r,c+1),(r+1,c),(r+1,c+1)}:
- *          the four bilinear taps of detector cell (r,c) in one vector load
+ *   QUAD : [views][rows+1][n_u-1] x 4 taps {(r,c),(r,c+1),(r+1,c),(r+1,c+1)}:
+ *          the four bilinear taps of detector cell (r,c), r = -1 .. rows-1,
+ *          in one vector load (rows -1 and `rows` read as zeros)
  *   PAIR : [rows][views][n_u-1] x 2 taps {(r,c),(r,c+1)} (fan beam) */
 typedef enum mct_layout { MCT_PLAIN = 0, MCT_QUAD = 1, MCT_PAIR = 2 } mct_layout;
 

@@ -16,7 +16,8 @@
 //
 // The filtered sinogram arrives pre-packed as bilinear coefficients, so the
 // taps of one update are a single 16-byte (FP32) or 8-byte (FP16/BF16) load:
-// QUAD layout [view][row-1][col-1] x {a, b-a, c-a, d-c-b+a} for cone beam,
+// QUAD layout [view][row+1][col-1] x {a, b-a, c-a, d-c-b+a} for cone beam
+// (cell rows -1 .. rows-1, rows beyond the detector read as zeros),
 // PAIR layout [row][view][col-1] x {a, b-a} for fan beam.  Workgroups are
 // remapped so that each XCD owns a contiguous range of slice chunks: its L2
 // then only holds the detector rows that range projects onto.
@@ -39,8 +40,10 @@ namespace {
 using mct::kBufferDword3;
 
 // Window of packed cells of one (tile, chunk, view) for the LDS-staged cone
-// kernel: first cell (column c0, row r0) and extent w x h; all inside
-// [0, n_u-2] x [0, n_v-2].
+// kernel: first cell (column c0, row r0) and extent w x h.  Columns lie in
+// [0, n_u-2]; rows are not clipped to the detector (r0 may be negative, and
+// r0 + h may pass n_v - 1), so every voxel of the tile and chunk, inside the
+// mask or not, has its row coordinate inside the window.
 struct BpWin {
     int16_t c0, r0, w, h;
 };
@@ -274,7 +277,7 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
     for (int k = 0; k < ZC; ++k) acc[k] = (a.accum && k < nk) ? out[(size_t)(z0 + k) * plane] : 0.f;
     int lo = 0, hi = nk - 1;
     const uint32_t rowbytes = (uint32_t)(a.nu - 1) * Taps<DT>::kQuadBytes;
-    const size_t viewbytes = (size_t)(a.nv - 1) * rowbytes;
+    const size_t viewbytes = (size_t)(a.nv + 1) * rowbytes;  // cell rows -1 .. nv - 1
     const char *sino = reinterpret_cast<const char *>(a.sino);
     const float4 *vt = a.vt + a.j0;
     for (int j = 0; j < a.nvb; ++j) {
@@ -291,9 +294,11 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
         lo = l > lo ? l : lo;
         hi = h < hi ? h : hi;
         if (lo > hi) break;
-        // one buffer resource per view (SGPRs): 32-bit per-lane offsets
+        // one buffer resource per view (SGPRs), based at cell row 0: 32-bit
+        // per-lane offsets
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(sino + (size_t)j * viewbytes), 0, (int)viewbytes, kBufferDword3);
+            (void *)(sino + (size_t)j * viewbytes + rowbytes), 0, (int)(viewbytes - rowbytes),
+            kBufferDword3);
         const uint32_t uoff = cv.uf * (uint32_t)Taps<DT>::kQuadBytes;
         // batches of B independent loads, issued before any of them is used
 #pragma unroll
@@ -376,15 +381,6 @@ __device__ __forceinline__ FastView column_view_fast(const BpArgs &a, float x, f
     return cv;
 }
 
-// float(v) for 0 <= v < 2^24 from integer operations only, so a wave-uniform
-// v stays on the scalar unit (which has no float conversion).
-__device__ __forceinline__ float small_int_to_float(int v) {
-    if (v <= 0) return 0.f;
-    const int e = 31 - __builtin_clz((unsigned)v);  // 2^e <= v < 2^(e+1), e <= 23
-    const unsigned m = ((unsigned)v << (23 - e)) & 0x7fffffu;
-    return __uint_as_float(((unsigned)(127 + e) << 23) | m);
-}
-
 // d = fl(fl(mag * z) - v[0]) of slice iz, the quantity of the exact v test.
 __device__ __forceinline__ float v_coord(const BpArgs &a, float mag, int iz, float halfz) {
     float z = ((float)iz - halfz) * a.vh + a.cz;
@@ -434,8 +430,11 @@ __global__ __launch_bounds__(256) void bp_column_iv_kernel(BpArgs a, int2 *__res
 }
 
 // Window table, FP64 corner evaluation; maxes[0..1] collect the largest w, h.
-// A tile with a corner at or behind the source (L <= 0) gets an oversized
-// window, which makes the launch fall back to the direct kernel.
+// The rows span every slice position of the chunk, kLdsZC of them even in a
+// last, partial chunk (the kernel computes those slices and discards them),
+// and are not clipped to the detector.  A tile with a corner at or behind
+// the source (L <= 0) gets an oversized window, which makes the launch fall
+// back to the direct kernel.
 __global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restrict__ win,
                                                        int *__restrict__ maxes) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -444,7 +443,7 @@ __global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restri
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int ixs[2] = {tx * kLdsTile, min(tx * kLdsTile + kLdsTile - 1, a.nx - 1)};
     const int iys[2] = {ty * kLdsTile, min(ty * kLdsTile + kLdsTile - 1, a.nx - 1)};
-    const int iz0 = zci * kLdsZC, iz1 = min(iz0 + kLdsZC - 1, a.nz - 1);
+    const int iz0 = zci * kLdsZC, iz1 = iz0 + kLdsZC - 1;
     const double half = (a.nx - 1) / 2.0, halfz = (a.nz - 1) / 2.0;
     const double c = a.vt[j].x, s = a.vt[j].y;
     const double zs[2] = {(iz0 - halfz) * a.vh + a.cz, (iz1 - halfz) * a.vh + a.cz};
@@ -471,17 +470,20 @@ __global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restri
             }
         }
     }
-    const double cmax = a.nu - 2, rmax = a.nv - 2;
+    const double cmax = a.nu - 2;
     const double cl = fmin(fmax(floor(umin) - 1.0, 0.0), cmax);
     const double ch = fmin(fmax(floor(umax) + 1.0, 0.0), cmax);
-    const double rl = fmin(fmax(floor(tmin) - 1.0, 0.0), rmax);
-    const double rh = fmin(fmax(floor(tmax) + 1.0, 0.0), rmax);
-    int w = ch >= cl ? (int)(ch - cl) + 1 : 0;
-    int h = rh >= rl ? (int)(rh - rl) + 1 : 0;
-    if (bad || !(umin <= umax) || !(tmin <= tmax)) w = h = 32767;
+    const double rl = floor(tmin) - 1.0;
+    const double rh = floor(tmax) + 1.0;
+    const bool fits = !bad && umin <= umax && tmin <= tmax && rl >= -32768.0 && rh <= 32767.0;
+    int w = 32767, h = 32767;
+    if (fits) {
+        w = ch >= cl ? (int)(ch - cl) + 1 : 0;
+        h = (int)fmin(rh - rl + 1.0, 32767.0);
+    }
     BpWin e;
     e.c0 = (int16_t)cl;
-    e.r0 = (int16_t)rl;
+    e.r0 = fits ? (int16_t)rl : (int16_t)0;
     e.w = (int16_t)w;
     e.h = (int16_t)h;
     win[((size_t)tile * a.nzc + zci) * a.nviews + j] = e;
@@ -531,39 +533,58 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
     if (__syncthreads_or(alive)) {
         const int tid = threadIdx.x;
         const uint32_t rowbytes = (uint32_t)(a.nu - 1) * T::kQuadBytes;
-        const size_t viewbytes = (size_t)(a.nv - 1) * rowbytes;
+        const size_t viewbytes = (size_t)(a.nv + 1) * rowbytes;  // cell rows -1 .. nv - 1
         const char *sino = reinterpret_cast<const char *>(a.sino);
-        // staging slot q of this thread: window entry e = tid + 256 q, i.e.
-        // (row e / pitch, column e % pitch); its byte offset from the window
-        // origin is the same in every view
-        int srow[NQ];
+        // staging slot q of this thread: entry e = tid + 256 q of the rows
+        // loaded, i.e. (row e / pitch, column e % pitch) from the first
+        // loaded cell; its byte offset from that cell is the same in every
+        // view
+        // (a slot is in the first n rows exactly when soff < n * rowbytes;
+        // slots past the buffer get an offset no row count reaches)
         uint32_t soff[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int e = tid + 256 * q;
             const int r = e / pitch, c = e - r * pitch;
-            srow[q] = e < cells ? r : 0x7fffffff;
-            soff[q] = (uint32_t)r * rowbytes + (uint32_t)c * T::kQuadBytes;
+            soff[q] = e < cells ? (uint32_t)r * rowbytes + (uint32_t)c * T::kQuadBytes : 0xffffffffu;
         }
         // (a whole-scan launch reads the fields as given: j0 = 0, nvb = nviews)
         const BpWin *wt = win + ((size_t)tile * a.nzc + zci) * a.nviews + (ACC ? a.j0 : 0);
         const float4 *vt = ACC ? a.vt + a.j0 : a.vt;
         const int nvb = ACC ? a.nvb : a.nviews;
+        // A window's rows [r0, r0 + h) are not clipped to the detector, so no
+        // row coordinate of the tile leaves it and the slice loop needs no
+        // clamp.  Of those rows, the packed cell rows [-1, nv - 1] (the
+        // detector's cells and the two zero-padded ones around them, where a
+        // voxel inside the mask can land by the rounding of its row
+        // coordinate) are loaded; other rows keep whatever the buffer held
+        // and are read only for voxels outside the mask, whose sums are
+        // discarded.
+        auto first_row = [&](const BpWin &w) { return w.r0 > -1 ? (int)w.r0 : -1; };
+        auto row_count = [&](const BpWin &w) {
+            const int end = (int)w.r0 + (int)w.h < a.nv ? (int)w.r0 + (int)w.h : a.nv;
+            return end > first_row(w) ? end - first_row(w) : 0;
+        };
         Raw stage[NQ];
         auto fetch = [&](int j, const BpWin &w) {
-            const size_t origin = (size_t)w.r0 * rowbytes + (size_t)w.c0 * T::kQuadBytes;
+            const uint32_t lim = (uint32_t)row_count(w) * rowbytes;
+            const size_t origin = (size_t)(first_row(w) + 1) * rowbytes + (size_t)w.c0 * T::kQuadBytes;
             const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(sino + (size_t)j * viewbytes + origin), 0, (int)(viewbytes - origin),
                 kBufferDword3);
+            // (slots past the rows to load skip their load: a range-checked
+            // load that returns zeros still occupies the address unit, and
+            // made the launch 3% slower)
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
-                if (srow[q] < w.h) stage[q] = T::load_buf(rsrc, soff[q]);
+                if (soff[q] < lim) stage[q] = T::load_buf(rsrc, soff[q]);
         };
         auto put = [&](int b, const BpWin &w) {
-            Raw *dst = lds + b * cells + tid;
+            const uint32_t lim = (uint32_t)row_count(w) * rowbytes;
+            Raw *dst = lds + b * cells + (first_row(w) - (int)w.r0) * pitch + tid;
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
-                if (srow[q] < w.h) dst[256 * q] = stage[q];
+                if (soff[q] < lim) dst[256 * q] = stage[q];
         };
         // Per-view table entries pass through LDS in batches of 64 views:
         // s_vt[i] = {cos, sin, dbeta, weight} of view j0 + i, s_win[i] = window
@@ -605,16 +626,13 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
             }
             if (alive) {
                 const FastView cv = column_view_fast(a, x, y, s_vt[j & 63]);
-                // row coordinate relative to the window's first row; the
-                // subtraction is exact when 0 <= r0 <= t0, which holds
-                // whenever a slice of the chunk passes the range tests
-                const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - small_int_to_float(wc.r0);
+                // row coordinate relative to the window's first row: in
+                // [0, h) for every slice of the chunk (the window spans the
+                // tile's rows with a cell of margin on either side)
+                const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - (float)(int)wc.r0;
                 const float dt = cv.mag * zstep_t;
                 const int wmax = wc.w > 0 ? wc.w - 1 : 0;
                 const int lc = min(max((int)cv.uf - (int)wc.c0, 0), wmax);
-                // largest float below h: a clamped row coordinate floors to <= h - 1
-                const float tlim =
-                    wc.h > 0 ? __uint_as_float(__float_as_uint(small_int_to_float(wc.h)) - 1u) : 0.f;
                 const unsigned char *src =
                     smem + (uint32_t)((j & 1) * cells + lc) * (uint32_t)sizeof(Raw);
                 // reads in batches of 4, all in flight before the first is used
@@ -628,8 +646,7 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
                     float rv[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        float tk = fmaf((float)(k0 + i), dt, t0);
-                        tk = __builtin_amdgcn_fmed3f(tk, 0.f, tlim);
+                        const float tk = fmaf((float)(k0 + i), dt, t0);
                         const uint32_t vf = (uint32_t)tk;
                         rv[i] = __builtin_amdgcn_fractf(tk);
                         q[i] = *reinterpret_cast<const Raw *>(src + __umul24(vf, row_stride));
@@ -831,7 +848,7 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
             int vf = (int)vff;
             float rv = vidx - vff;
             if (a.layout == MCT_QUAD) {
-                float4 q = Taps<DT>::quad(a.sino, ((size_t)jl * (a.nv - 1) + vf) * ncol + uf);
+                float4 q = Taps<DT>::quad(a.sino, ((size_t)jl * (a.nv + 1) + vf + 1) * ncol + uf);
                 acc += coef * bilerp(q, ru, rv);
             } else {
                 size_t r0 = ((size_t)vf * a.nvb + jl) * a.nu + uf;

@@ -82,30 +82,40 @@ template <int DT> __device__ __forceinline__ uint16_t to_half_bits(float f) {
     return DT == MCT_F16 ? mct::f32_to_f16_rn(f) : mct::f32_to_bf16_rn(f);
 }
 
-// QUAD: out[j][r][c] = bilinear coefficients of cell (r, c) of view j:
-// {a, b - a, c - a, d - c - b + a} with a = in[r][j][c], b = in[r][j][c+1],
-// c = in[r+1][j][c], d = in[r+1][j][c+1] (each rounded once from FP64).
-// A thread walks kQuadRows cell rows of one column, so every detector row it
-// reads serves two cells.
+// QUAD: out[j][r + 1][c] = bilinear coefficients of cell (r, c) of view j,
+// r = -1 .. rows - 1: {a, b - a, c - a, d - c - b + a} with a = in[r][j][c],
+// b = in[r][j][c+1], c = in[r+1][j][c], d = in[r+1][j][c+1] (each rounded
+// once from FP64), rows -1 and `rows` read as zeros.  The two cells beyond
+// the detector's rows let a back-projector interpolate any row coordinate in
+// [-1, rows) without clamping it.  A thread walks kQuadRows cell rows of one
+// column, so every detector row it reads serves two cells.
 constexpr int kQuadRows = 16;
 template <int DT>
 __global__ void pack_quad_kernel(const float *__restrict__ in, int rows, int views, int nu,
                                  void *__restrict__ out) {
     // grid: x over detector cells c, y over views j, z over chunks of cell rows
-    const int nc = nu - 1, nr = rows - 1;
+    const int nc = nu - 1, ns = rows + 1;  // cell rows s = r + 1 in [0, rows]
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, rb = blockIdx.z * kQuadRows;
+    const int j = blockIdx.y, sb = blockIdx.z * kQuadRows;
     if (c >= nc) return;
-    const int re = rb + kQuadRows < nr ? rb + kQuadRows : nr;
+    const int se = sb + kQuadRows < ns ? sb + kQuadRows : ns;
     const int64_t rstride = (int64_t)views * nu;
-    const float *p = in + ((int64_t)rb * views + j) * nu + c;
-    double a = p[0], b = p[1];
-    for (int r = rb; r < re; ++r) {
-        p += rstride;
-        const double cc = p[0], d = p[1];
+    const float *p = in + (int64_t)j * nu + c;  // detector row 0
+    // cell s = sr spans detector rows sr - 1 (a, b) and sr (c, d)
+    double a = 0.0, b = 0.0;
+    if (sb >= 1) {
+        a = p[(sb - 1) * rstride];
+        b = p[(sb - 1) * rstride + 1];
+    }
+    for (int sr = sb; sr < se; ++sr) {
+        double cc = 0.0, d = 0.0;
+        if (sr < rows) {
+            cc = p[sr * rstride];
+            d = p[sr * rstride + 1];
+        }
         // bilinear coefficients: v(ru, rv) = q0 + ru q1 + rv (q2 + ru q3)
         const float q0 = (float)a, q1 = (float)(b - a), q2 = (float)(cc - a), q3 = (float)(d - cc - b + a);
-        const size_t idx = ((size_t)j * nr + r) * nc + c;
+        const size_t idx = ((size_t)j * ns + sr) * nc + c;
         if (DT == MCT_F32) {
             reinterpret_cast<float4 *>(out)[idx] = make_float4(q0, q1, q2, q3);
         } else {
@@ -206,7 +216,7 @@ int64_t mct_packed_bytes(int32_t rows, int32_t views, int32_t n_u, int32_t dtype
         case MCT_PLAIN: return (int64_t)rows * views * n_u * es;
         case MCT_QUAD:
             if (rows < 2 || n_u < 2) return -1;
-            return (int64_t)(rows - 1) * views * (n_u - 1) * 4 * es;
+            return (int64_t)(rows + 1) * views * (n_u - 1) * 4 * es;
         case MCT_PAIR:
             if (n_u < 2) return -1;
             return (int64_t)rows * views * (n_u - 1) * 2 * es;
@@ -224,7 +234,7 @@ int mct_pack_sinogram(mct_ctx *ctx, const float *in, int32_t rows, int32_t views
     MCT_CHECK(rows <= 65535 && views <= 65535, "rows and views must be <= 65535");
     dim3 grid;
     if (layout == MCT_QUAD)
-        grid = dim3(mct::blocks_for(n_u - 1, 256), views, (rows - 1 + kQuadRows - 1) / kQuadRows);
+        grid = dim3(mct::blocks_for(n_u - 1, 256), views, (rows + 1 + kQuadRows - 1) / kQuadRows);
     else if (layout == MCT_PAIR)
         grid = dim3(mct::blocks_for(n_u - 1, 256), rows, views);
     int64_t n = (int64_t)rows * views * n_u;

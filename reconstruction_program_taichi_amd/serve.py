@@ -78,7 +78,7 @@ class Job:
         if self.kind == "cgls":
             return sg * 3 + vol * 7
         tap = 4 if self.storage == "f32" else 2
-        packed = max(s.n_v - 1, 1) * s.n_views * (s.n_u - 1) * 4 * tap
+        packed = (s.n_v + 1) * s.n_views * (s.n_u - 1) * 4 * tap
         return sg * 2 + packed + vol
 
 
