@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 9
+#define MCT_ABI_VERSION 10
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -42,8 +42,11 @@ typedef enum mct_dtype { MCT_F32 = 0, MCT_F16 = 1, MCT_BF16 = 2 } mct_dtype;
  *   QUAD : [views][rows+1][n_u-1] x 4 taps {(r,c),(r,c+1),(r+1,c),(r+1,c+1)}:
  *          the four bilinear taps of detector cell (r,c), r = -1 .. rows-1,
  *          in one vector load (rows -1 and `rows` read as zeros)
- *   PAIR : [rows][views][n_u-1] x 2 taps {(r,c),(r,c+1)} (fan beam) */
-typedef enum mct_layout { MCT_PLAIN = 0, MCT_QUAD = 1, MCT_PAIR = 2 } mct_layout;
+ *   PAIR : [rows][views][n_u-1] x 2 taps {(r,c),(r,c+1)} (fan beam)
+ *   ROWS : [views][rows+2][n_u-1] x {(r,c), (r,c+1) - (r,c)}, r = -1 .. rows
+ *          (rows -1 and `rows` read as zeros): cone beam, half the bytes of
+ *          QUAD; the back-projector forms a cell's four taps from two rows */
+typedef enum mct_layout { MCT_PLAIN = 0, MCT_QUAD = 1, MCT_PAIR = 2, MCT_ROWS = 3 } mct_layout;
 
 /* ramp-filter arithmetic */
 typedef enum mct_filter_method {

@@ -18,13 +18,13 @@ MCT_ERR_UNSUPPORTED = -3
 MCT_ERR_NOMEM = -4
 
 MCT_F32, MCT_F16, MCT_BF16 = 0, 1, 2
-MCT_PLAIN, MCT_QUAD, MCT_PAIR = 0, 1, 2
+MCT_PLAIN, MCT_QUAD, MCT_PAIR, MCT_ROWS = 0, 1, 2, 3
 MCT_FILTER_F32_DIRECT, MCT_FILTER_BF16_MFMA, MCT_FILTER_BF16X3_MFMA = 0, 1, 2
 
 MCT_BP_AUTO, MCT_BP_DIRECT, MCT_BP_LDS = 0, 1, 2
 MCT_FP_AUTO, MCT_FP_DIRECT, MCT_FP_CELLS = 0, 1, 2
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 
 class MctError(RuntimeError):

@@ -106,6 +106,15 @@ template <> struct Taps<MCT_F32> {
     __device__ __forceinline__ static float one(const void *base, size_t i) {
         return reinterpret_cast<const float *>(base)[i];
     }
+    // ROWS: the taps {a, b-a} of one row element, and the QUAD cell of two
+    // vertically neighbouring ones: {a, b-a, c-a, (d-c)-(b-a)}
+    using Pair = float2;
+    __device__ __forceinline__ static Pair load_pair(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    __device__ __forceinline__ static Raw quad_of(Pair p0, Pair p1) {
+        return make_float4(p0.x, p0.y, p1.x - p0.x, p1.y - p0.y);
+    }
 };
 template <> struct Taps<MCT_F16> {
     static constexpr int kQuadBytes = 8, kPairBytes = 4;
@@ -133,6 +142,16 @@ template <> struct Taps<MCT_F16> {
     __device__ __forceinline__ static float one(const void *base, size_t i) {
         return mct::f16_to_f32(reinterpret_cast<const uint16_t *>(base)[i]);
     }
+    using Pair = uint32_t;  // {a, b-a} as two FP16 halves, a in the low one
+    __device__ __forceinline__ static Pair load_pair(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    }
+    __device__ __forceinline__ static Raw quad_of(Pair p0, Pair p1) {
+        const uint16_t a0 = (uint16_t)p0, d0 = (uint16_t)(p0 >> 16);
+        const uint16_t a1 = (uint16_t)p1, d1 = (uint16_t)(p1 >> 16);
+        return make_ushort4(a0, d0, mct::f32_to_f16_rn(mct::f16_to_f32(a1) - mct::f16_to_f32(a0)),
+                            mct::f32_to_f16_rn(mct::f16_to_f32(d1) - mct::f16_to_f32(d0)));
+    }
 };
 template <> struct Taps<MCT_BF16> {
     static constexpr int kQuadBytes = 8, kPairBytes = 4;
@@ -159,6 +178,15 @@ template <> struct Taps<MCT_BF16> {
     }
     __device__ __forceinline__ static float one(const void *base, size_t i) {
         return mct::bf16_to_f32(reinterpret_cast<const uint16_t *>(base)[i]);
+    }
+    using Pair = uint32_t;  // {a, b-a} as two BF16 halves, a in the low one
+    __device__ __forceinline__ static Pair load_pair(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    }
+    __device__ __forceinline__ static Raw quad_of(Pair p0, Pair p1) {
+        const float c = __uint_as_float(p1 << 16) - __uint_as_float(p0 << 16);
+        const float e = __uint_as_float(p1 & 0xffff0000u) - __uint_as_float(p0 & 0xffff0000u);
+        return make_uint2(p0, (uint32_t)mct::f32_to_bf16_rn(c) | ((uint32_t)mct::f32_to_bf16_rn(e) << 16));
     }
 };
 
@@ -257,7 +285,7 @@ __device__ __forceinline__ void v_interval(const BpArgs &a, float mag, int z0, i
 
 // Cone beam, axial view, QUAD taps; ZC slices per thread, loads in batches
 // of B.
-template <int ZC, int B, int DT>
+template <int ZC, int B, int DT, bool ROWS>
 __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
     static_assert(ZC % B == 0, "batch must divide the slice chunk");
     int ix, iy, zci;
@@ -276,8 +304,9 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
 #pragma unroll
     for (int k = 0; k < ZC; ++k) acc[k] = (a.accum && k < nk) ? out[(size_t)(z0 + k) * plane] : 0.f;
     int lo = 0, hi = nk - 1;
-    const uint32_t rowbytes = (uint32_t)(a.nu - 1) * Taps<DT>::kQuadBytes;
-    const size_t viewbytes = (size_t)(a.nv + 1) * rowbytes;  // cell rows -1 .. nv - 1
+    // QUAD: cell rows -1 .. nv - 1; ROWS: element rows -1 .. nv (PAIR taps)
+    const uint32_t rowbytes = (uint32_t)(a.nu - 1) * (ROWS ? Taps<DT>::kPairBytes : Taps<DT>::kQuadBytes);
+    const size_t viewbytes = (size_t)(a.nv + (ROWS ? 2 : 1)) * rowbytes;
     const char *sino = reinterpret_cast<const char *>(a.sino);
     const float4 *vt = a.vt + a.j0;
     for (int j = 0; j < a.nvb; ++j) {
@@ -299,23 +328,45 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(sino + (size_t)j * viewbytes + rowbytes), 0, (int)(viewbytes - rowbytes),
             kBufferDword3);
-        const uint32_t uoff = cv.uf * (uint32_t)Taps<DT>::kQuadBytes;
         // batches of B independent loads, issued before any of them is used
+        if constexpr (ROWS) {  // the two rows of a cell: two PAIR loads
+            const uint32_t uoff = cv.uf * (uint32_t)Taps<DT>::kPairBytes;
 #pragma unroll
-        for (int k0 = 0; k0 < ZC; k0 += B) {
-            typename Taps<DT>::Raw raw[B];
-            float rv[B];
+            for (int k0 = 0; k0 < ZC; k0 += B) {
+                typename Taps<DT>::Pair p0[B], p1[B];
+                float rv[B];
 #pragma unroll
-            for (int i = 0; i < B; ++i) {
-                float tk = fmaf((float)(k0 + i), dt, t0);
-                tk = __builtin_amdgcn_fmed3f(tk, 0.f, a.vlim);
-                uint32_t vf = (uint32_t)tk;
-                rv[i] = __builtin_amdgcn_fractf(tk);
-                raw[i] = Taps<DT>::load_buf(rsrc, __umul24(vf, rowbytes) + uoff);
+                for (int i = 0; i < B; ++i) {
+                    float tk = fmaf((float)(k0 + i), dt, t0);
+                    tk = __builtin_amdgcn_fmed3f(tk, 0.f, a.vlim);
+                    uint32_t vf = (uint32_t)tk;
+                    rv[i] = __builtin_amdgcn_fractf(tk);
+                    p0[i] = Taps<DT>::load_pair(rsrc, __umul24(vf, rowbytes) + uoff);
+                    p1[i] = Taps<DT>::load_pair(rsrc, __umul24(vf, rowbytes) + rowbytes + uoff);
+                }
+#pragma unroll
+                for (int i = 0; i < B; ++i)
+                    acc[k0 + i] = fmaf(cv.coef, bilerp(Taps<DT>::cvt(Taps<DT>::quad_of(p0[i], p1[i])), cv.ru, rv[i]),
+                                       acc[k0 + i]);
             }
+        } else {
+            const uint32_t uoff = cv.uf * (uint32_t)Taps<DT>::kQuadBytes;
 #pragma unroll
-            for (int i = 0; i < B; ++i)
-                acc[k0 + i] = fmaf(cv.coef, bilerp(Taps<DT>::cvt(raw[i]), cv.ru, rv[i]), acc[k0 + i]);
+            for (int k0 = 0; k0 < ZC; k0 += B) {
+                typename Taps<DT>::Raw raw[B];
+                float rv[B];
+#pragma unroll
+                for (int i = 0; i < B; ++i) {
+                    float tk = fmaf((float)(k0 + i), dt, t0);
+                    tk = __builtin_amdgcn_fmed3f(tk, 0.f, a.vlim);
+                    uint32_t vf = (uint32_t)tk;
+                    rv[i] = __builtin_amdgcn_fractf(tk);
+                    raw[i] = Taps<DT>::load_buf(rsrc, __umul24(vf, rowbytes) + uoff);
+                }
+#pragma unroll
+                for (int i = 0; i < B; ++i)
+                    acc[k0 + i] = fmaf(cv.coef, bilerp(Taps<DT>::cvt(raw[i]), cv.ru, rv[i]), acc[k0 + i]);
+            }
         }
     }
 #pragma unroll
@@ -323,9 +374,9 @@ __device__ __forceinline__ void cone_tile(const BpArgs &a, int block) {
         if (k < nk) out[(size_t)(z0 + k) * plane] = bp_result(a, k >= lo && k <= hi, acc[k]);
 }
 
-template <int ZC, int B, int DT>
+template <int ZC, int B, int DT, bool ROWS>
 __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
-    cone_tile<ZC, B, DT>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd));
+    cone_tile<ZC, B, DT, ROWS>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd));
 }
 
 // ---------------------------------------------------------------------------
@@ -497,7 +548,7 @@ __global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restri
 // packed cells; ACC: the launch may be a view block (a.accum != 0), else the
 // volume is overwritten.  Every thread of the workgroup must call this
 // (barriers).
-template <int DT, int NQ, bool ACC>
+template <int DT, int NQ, bool ACC, bool ROWS>
 __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsigned char *smem) {
     using T = Taps<DT>;
     using Raw = typename T::Raw;
@@ -532,22 +583,10 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
 
     if (__syncthreads_or(alive)) {
         const int tid = threadIdx.x;
-        const uint32_t rowbytes = (uint32_t)(a.nu - 1) * T::kQuadBytes;
-        const size_t viewbytes = (size_t)(a.nv + 1) * rowbytes;  // cell rows -1 .. nv - 1
+        // QUAD: cell rows -1 .. nv - 1; ROWS: element rows -1 .. nv
+        const uint32_t rowbytes = (uint32_t)(a.nu - 1) * (ROWS ? T::kPairBytes : T::kQuadBytes);
+        const size_t viewbytes = (size_t)(a.nv + (ROWS ? 2 : 1)) * rowbytes;
         const char *sino = reinterpret_cast<const char *>(a.sino);
-        // staging slot q of this thread: entry e = tid + 256 q of the rows
-        // loaded, i.e. (row e / pitch, column e % pitch) from the first
-        // loaded cell; its byte offset from that cell is the same in every
-        // view
-        // (a slot is in the first n rows exactly when soff < n * rowbytes;
-        // slots past the buffer get an offset no row count reaches)
-        uint32_t soff[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int e = tid + 256 * q;
-            const int r = e / pitch, c = e - r * pitch;
-            soff[q] = e < cells ? (uint32_t)r * rowbytes + (uint32_t)c * T::kQuadBytes : 0xffffffffu;
-        }
         // (a whole-scan launch reads the fields as given: j0 = 0, nvb = nviews)
         const BpWin *wt = win + ((size_t)tile * a.nzc + zci) * a.nviews + (ACC ? a.j0 : 0);
         const float4 *vt = ACC ? a.vt + a.j0 : a.vt;
@@ -565,26 +604,88 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
             const int end = (int)w.r0 + (int)w.h < a.nv ? (int)w.r0 + (int)w.h : a.nv;
             return end > first_row(w) ? end - first_row(w) : 0;
         };
-        Raw stage[NQ];
-        auto fetch = [&](int j, const BpWin &w) {
-            const uint32_t lim = (uint32_t)row_count(w) * rowbytes;
-            const size_t origin = (size_t)(first_row(w) + 1) * rowbytes + (size_t)w.c0 * T::kQuadBytes;
-            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(sino + (size_t)j * viewbytes + origin), 0, (int)(viewbytes - origin),
-                kBufferDword3);
-            // (slots past the rows to load skip their load: a range-checked
-            // load that returns zeros still occupies the address unit, and
-            // made the launch 3% slower)
+        // QUAD: staging slot q of this thread is entry e = tid + 256 q of the
+        // rows loaded, i.e. (row e / pitch, column e % pitch) from the first
+        // loaded cell; its byte offset from that cell is the same in every
+        // view (a slot is in the first n rows exactly when soff < n *
+        // rowbytes; slots past the buffer get an offset no row count
+        // reaches).
+        // ROWS: strip q of this thread (NQ / 2 of them) is s = tid + 256 q:
+        // window column s % pitch, cell rows 4 (s / pitch) .. + 3; it loads
+        // the five element rows of its four cells (8 B each for FP32: 10 B
+        // per cell against QUAD's 16) and forms the cells in registers.
+        constexpr int NS = ROWS ? NQ / 2 : 1;
+        constexpr int NSLOT = ROWS ? 1 : NQ;
+        uint32_t soff[NSLOT];
+        uint32_t goff[NS];
+        int lidx[NS], srow0[NS];
+        if constexpr (ROWS) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                if (soff[q] < lim) stage[q] = T::load_buf(rsrc, soff[q]);
+            for (int q = 0; q < NS; ++q) {
+                const int e = tid + 256 * q;
+                const int g = e / pitch, c = e - g * pitch;
+                srow0[q] = 4 * g < rows ? 4 * g : 0x7fffffff;
+                goff[q] = (uint32_t)(4 * g) * rowbytes + (uint32_t)c * T::kPairBytes;
+                lidx[q] = 4 * g * pitch + c;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int e = tid + 256 * q;
+                const int r = e / pitch, c = e - r * pitch;
+                soff[q] = e < cells ? (uint32_t)r * rowbytes + (uint32_t)c * T::kQuadBytes : 0xffffffffu;
+            }
+        }
+        Raw stage[NSLOT];
+        typename T::Pair pst[NS][5];
+        auto fetch = [&](int j, const BpWin &w) {
+            const int nr = row_count(w);
+            // first loaded row: stored row first_row + 1 of the view
+            const size_t origin = (size_t)(first_row(w) + 1) * rowbytes +
+                                  (size_t)w.c0 * (ROWS ? T::kPairBytes : T::kQuadBytes);
+            if constexpr (ROWS) {
+                // the resource ends after element row nr (the last the loaded
+                // cells read): loads past it return zeros
+                const size_t rest = viewbytes - origin, need = (size_t)(nr + 1) * rowbytes;
+                const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(sino + (size_t)j * viewbytes + origin), 0, (int)(need < rest ? need : rest),
+                    kBufferDword3);
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    if (srow0[q] < nr) {
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) pst[q][k] = T::load_pair(rsrc, goff[q] + (uint32_t)k * rowbytes);
+                    }
+            } else {
+                const uint32_t lim = (uint32_t)nr * rowbytes;
+                const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(sino + (size_t)j * viewbytes + origin), 0, (int)(viewbytes - origin),
+                    kBufferDword3);
+                // (slots past the rows to load skip their load: a range-checked
+                // load that returns zeros still occupies the address unit, and
+                // made the launch 3% slower)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    if (soff[q] < lim) stage[q] = T::load_buf(rsrc, soff[q]);
+            }
         };
         auto put = [&](int b, const BpWin &w) {
-            const uint32_t lim = (uint32_t)row_count(w) * rowbytes;
-            Raw *dst = lds + b * cells + (first_row(w) - (int)w.r0) * pitch + tid;
+            const int nr = row_count(w);
+            Raw *buf = lds + b * cells + (first_row(w) - (int)w.r0) * pitch;
+            if constexpr (ROWS) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                if (soff[q] < lim) dst[256 * q] = stage[q];
+                for (int q = 0; q < NS; ++q)
+                    if (srow0[q] < nr) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (srow0[q] + k < nr) buf[lidx[q] + k * pitch] = T::quad_of(pst[q][k], pst[q][k + 1]);
+                    }
+            } else {
+                const uint32_t lim = (uint32_t)nr * rowbytes;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    if (soff[q] < lim) buf[tid + 256 * q] = stage[q];
+            }
         };
         // Per-view table entries pass through LDS in batches of 64 views:
         // s_vt[i] = {cos, sin, dbeta, weight} of view j0 + i, s_win[i] = window
@@ -673,10 +774,10 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
     }
 }
 
-template <int DT, int NQ, bool ACC>
+template <int DT, int NQ, bool ACC, bool ROWS>
 __global__ __launch_bounds__(256) void bp_cone_lds_kernel(BpArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cone_lds_tile<DT, NQ, ACC>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
+    cone_lds_tile<DT, NQ, ACC, ROWS>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
                                smem);
 }
 
@@ -752,12 +853,17 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict_
         const int local = item - offsets[lo];
         if constexpr (NQ > 0) {
             if (a.cone && a.win) {
-                cone_lds_tile<DT, NQ, false>(a, local, smem);
+                if (a.layout == MCT_ROWS)
+                    cone_lds_tile<DT, NQ, false, true>(a, local, smem);
+                else
+                    cone_lds_tile<DT, NQ, false, false>(a, local, smem);
                 continue;
             }
         }
-        if (a.cone)
-            cone_tile<16, 16, DT>(a, local);
+        if (a.cone && a.layout == MCT_ROWS)
+            cone_tile<16, 16, DT, true>(a, local);
+        else if (a.cone)
+            cone_tile<16, 16, DT, false>(a, local);
         else
             fan_tile<8, DT>(a, local);
     }
@@ -849,6 +955,11 @@ __global__ __launch_bounds__(256) void bp_generic_kernel(BpArgs a) {
             float rv = vidx - vff;
             if (a.layout == MCT_QUAD) {
                 float4 q = Taps<DT>::quad(a.sino, ((size_t)jl * (a.nv + 1) + vf + 1) * ncol + uf);
+                acc += coef * bilerp(q, ru, rv);
+            } else if (a.layout == MCT_ROWS) {
+                const size_t r0 = ((size_t)jl * (a.nv + 2) + vf + 1) * ncol + uf;
+                const typename Taps<DT>::Pair *pp = reinterpret_cast<const typename Taps<DT>::Pair *>(a.sino);
+                float4 q = Taps<DT>::cvt(Taps<DT>::quad_of(pp[r0], pp[r0 + ncol]));
                 acc += coef * bilerp(q, ru, rv);
             } else {
                 size_t r0 = ((size_t)vf * a.nvb + jl) * a.nu + uf;
@@ -1257,6 +1368,18 @@ inline int lds_slots(int cells) {
 inline size_t lds_bytes(int cells, int32_t dtype) {
     return 2 * (size_t)cells * (dtype == MCT_F32 ? 16 : 8);
 }
+// The instantiated NQ for a pitch x rows window read from `layout`: QUAD
+// needs NQ * 256 staging slots >= the window's cells, ROWS NQ / 2 * 256
+// strips of 4 cell rows >= pitch * ceil(rows / 4) (as few as that: each
+// strip holds five element rows in registers); -1: none fits.
+inline int lds_nq(int pitch, int rows, int32_t layout) {
+    if (layout == MCT_ROWS) {
+        const int64_t strips = (int64_t)pitch * ((rows + 3) / 4);
+        const int64_t ns = (strips + 255) / 256;
+        return ns <= 4 ? (int)(2 * ns) : -1;
+    }
+    return lds_slots(pitch * rows);
+}
 // Automatic selection declines windows whose two LDS buffers exceed this
 // (occupancy would fall below 4 workgroups per CU).
 constexpr size_t kLdsAutoBytes = 40 * 1024;
@@ -1276,6 +1399,7 @@ int prepare_cone_lds(mct_plan *p, BpArgs &a, int32_t dtype, bool automatic) {
     if (rc) return rc;
     if (automatic && lds_bytes(p->bp_win_w * p->bp_win_h, dtype) > kLdsAutoBytes)
         return MCT_ERR_UNSUPPORTED;
+    if (lds_nq(p->bp_win_w, p->bp_win_h, a.layout) < 0) return MCT_ERR_UNSUPPORTED;
     a.win = reinterpret_cast<const BpWin *>(p->bp_win);
     a.iv = p->bp_iv;
     a.pitch = p->bp_win_w;
@@ -1290,20 +1414,25 @@ int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
     const size_t shmem = lds_bytes(cells, dtype);
     const int nblocks = a.tiles_x * a.tiles_y * a.nzc;
     mct_ctx *ctx = p->ctx;
-#define LDS_LAUNCH(DT, NQ)                                                                          \
-    do {                                                                                            \
-        if (a.accum)                                                                                \
-            hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ, true>), dim3(nblocks), dim3(256), shmem, \
-                               ctx->stream, a);                                                     \
-        else                                                                                        \
-            hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ, false>), dim3(nblocks), dim3(256), shmem, \
-                               ctx->stream, a);                                                     \
+#define LDS_LAUNCH2(DT, NQ, ROWS)                                                                 \
+    do {                                                                                              \
+        if (a.accum)                                                                                  \
+            hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ, true, ROWS>), dim3(nblocks), dim3(256),    \
+                               shmem, ctx->stream, a);                                                \
+        else                                                                                          \
+            hipLaunchKernelGGL((bp_cone_lds_kernel<DT, NQ, false, ROWS>), dim3(nblocks), dim3(256),   \
+                               shmem, ctx->stream, a);                                                \
+    } while (0)
+#define LDS_LAUNCH(DT, NQ)                                    \
+    do {                                                      \
+        if (a.layout == MCT_ROWS) LDS_LAUNCH2(DT, NQ, true);  \
+        else LDS_LAUNCH2(DT, NQ, false);                      \
     } while (0)
 #define LDS_DT(NQ)                                         \
     if (dtype == MCT_F32) LDS_LAUNCH(MCT_F32, NQ);         \
     else if (dtype == MCT_F16) LDS_LAUNCH(MCT_F16, NQ);    \
     else LDS_LAUNCH(MCT_BF16, NQ);
-    switch (lds_slots(cells)) {
+    switch (lds_nq(a.pitch, a.rows, a.layout)) {
         case 2: LDS_DT(2) break;
         case 4: LDS_DT(4) break;
         case 6: LDS_DT(6) break;
@@ -1311,6 +1440,7 @@ int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
     }
 #undef LDS_DT
 #undef LDS_LAUNCH
+#undef LDS_LAUNCH2
     MCT_LAUNCHED();
     return MCT_OK;
 }
@@ -1323,10 +1453,11 @@ constexpr int kMaxBatchJobs = 4096;
 // tile where the context allows it (building their plans' tables on first
 // use); lds_cells is the largest LDS window among them (0: none).
 int fill_batch(mct_ctx *ctx, const mct_bp_job *jobs, int njobs, int dtype, BpArgs *args, int *offs,
-               int &total_out, int &lds_cells) {
+               int &total_out, int &lds_cells, int &lds_q) {
     MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
     int64_t total = 0;
     lds_cells = 0;
+    lds_q = 0;
     for (int i = 0; i < njobs; ++i) {
         const mct_bp_job &jb = jobs[i];
         MCT_CHECK(jb.plan && jb.filt && jb.vol, "job %d: NULL plan, sinogram or volume", i);
@@ -1335,7 +1466,8 @@ int fill_batch(mct_ctx *ctx, const mct_bp_job *jobs, int njobs, int dtype, BpArg
         const mct_geometry &g = jb.plan->g;
         MCT_CHECK(g.view_mode == 1 && !g.apply_pmatrix && !g.curved,
                   "job %d: batched back-projection supports flat-detector axial jobs", i);
-        MCT_CHECK(jb.layout == (g.cone ? MCT_QUAD : MCT_PAIR), "job %d: cone reads QUAD, fan PAIR", i);
+        MCT_CHECK(g.cone ? (jb.layout == MCT_QUAD || jb.layout == MCT_ROWS) : jb.layout == MCT_PAIR,
+                  "job %d: cone reads QUAD or ROWS, fan PAIR", i);
         if (!g.cone) MCT_CHECK(g.n_v == g.img_dim_z, "job %d: fan beam needs one row per slice", i);
         if (g.cone) MCT_CHECK(g.det_h > 0.f, "job %d: cone beam needs det_h > 0", i);
         BpArgs a = make_args(jb.plan);
@@ -1350,8 +1482,9 @@ int fill_batch(mct_ctx *ctx, const mct_bp_job *jobs, int njobs, int dtype, BpArg
             const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
             int rc = prepare_cone_lds(jb.plan, a, dtype, automatic);
             if (rc == MCT_OK) {
-                const int cells = a.pitch * a.rows;
+                const int cells = a.pitch * a.rows, nq = lds_nq(a.pitch, a.rows, a.layout);
                 lds_cells = cells > lds_cells ? cells : lds_cells;
+                lds_q = nq > lds_q ? nq : lds_q;
             } else if (rc != MCT_ERR_UNSUPPORTED) {
                 return rc;
             } else if (!automatic) {
@@ -1373,9 +1506,9 @@ int fill_batch(mct_ctx *ctx, const mct_bp_job *jobs, int njobs, int dtype, BpArg
 // stream: as many workgroups as stay resident together.  Launches only (no
 // copy, allocation or synchronisation), so it can be captured in a graph.
 int launch_batch(mct_ctx *ctx, const BpArgs *dargs, const int *doffs, int njobs, int total, int dtype,
-                 int lds_cells) {
+                 int lds_cells, int lds_q) {
     const size_t shmem = lds_cells ? lds_bytes(lds_cells, dtype) : 0;
-    const int nq = lds_cells ? lds_slots(lds_cells) : 0;
+    const int nq = lds_cells ? lds_q : 0;
 #define BATCH_KERNEL(DT, NQ) bp_batch_kernel<DT, NQ>
 #define BATCH_LAUNCH(DT, NQ)                                                                        \
     do {                                                                                            \
@@ -1415,8 +1548,8 @@ int backproject(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
     mct_ctx *ctx = plan->ctx;
     MCT_CHECK(dtype == MCT_F32 || dtype == MCT_F16 || dtype == MCT_BF16, "bad dtype %d", dtype);
     if (g.cone)
-        MCT_CHECK(layout == MCT_QUAD || layout == MCT_PLAIN,
-                  "cone-beam back-projection reads QUAD or PLAIN taps (got %d)", layout);
+        MCT_CHECK(layout == MCT_QUAD || layout == MCT_ROWS || layout == MCT_PLAIN,
+                  "cone-beam back-projection reads QUAD, ROWS or PLAIN taps (got %d)", layout);
     else
         MCT_CHECK(layout == MCT_PAIR || layout == MCT_PLAIN,
                   "fan-beam back-projection reads PAIR or PLAIN taps (got %d)", layout);
@@ -1433,7 +1566,7 @@ int backproject(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
     a.nvb = nvb;
     a.accum = accum;
     const bool fast = g.view_mode == 1 && !g.apply_pmatrix && layout != MCT_PLAIN;
-    if (g.cone && layout == MCT_QUAD && ctx->bp_kernel != MCT_BP_DIRECT) {
+    if (g.cone && (layout == MCT_QUAD || layout == MCT_ROWS) && ctx->bp_kernel != MCT_BP_DIRECT) {
         const bool automatic = ctx->bp_kernel == MCT_BP_AUTO;
         int rc = launch_cone_lds(plan, a, dtype, automatic);
         if (rc == MCT_OK) {
@@ -1455,16 +1588,19 @@ int backproject(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
         if (g.cone) {
             int bt = ctx->bp_batch ? ctx->bp_batch : 16;
             bt = bt > zc ? zc : bt;
-#define CONE(ZC, B)                                                                            \
-    if (dtype == MCT_F32)                                                                      \
-        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_F32>), dim3(nblocks), dim3(256), 0, \
-                           ctx->stream, a);                                                    \
-    else if (dtype == MCT_F16)                                                                 \
-        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_F16>), dim3(nblocks), dim3(256), 0, \
-                           ctx->stream, a);                                                    \
-    else                                                                                       \
-        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_BF16>), dim3(nblocks), dim3(256), 0, \
+#define CONE_L(ZC, B, L)                                                                          \
+    if (dtype == MCT_F32)                                                                         \
+        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_F32, L>), dim3(nblocks), dim3(256), 0,  \
+                           ctx->stream, a);                                                       \
+    else if (dtype == MCT_F16)                                                                    \
+        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_F16, L>), dim3(nblocks), dim3(256), 0,  \
+                           ctx->stream, a);                                                       \
+    else                                                                                          \
+        hipLaunchKernelGGL((bp_cone_axial_kernel<ZC, B, MCT_BF16, L>), dim3(nblocks), dim3(256), 0, \
                            ctx->stream, a);
+#define CONE(ZC, B)                        \
+    if (layout == MCT_ROWS) { CONE_L(ZC, B, true) } \
+    else { CONE_L(ZC, B, false) }
             if (zc == 4) { CONE(4, 4) }
             else if (zc == 8 && bt == 4) { CONE(8, 4) }
             else if (zc == 8) { CONE(8, 8) }
@@ -1475,6 +1611,7 @@ int backproject(mct_plan *plan, const void *filt, int32_t dtype, int32_t layout,
             else if (bt == 8) { CONE(32, 8) }
             else { CONE(32, 16) }
 #undef CONE
+#undef CONE_L
         } else {
 #define CASE_DT(ZC)                                                                  \
     if (dtype == MCT_F32) MCT_AXIAL_LAUNCH(bp_fan_axial_kernel, ZC, MCT_F32);        \
@@ -1625,13 +1762,13 @@ int mct_backproject_batch(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs) {
     }
     BpArgs *args = reinterpret_cast<BpArgs *>(b.host);
     int *offs = reinterpret_cast<int *>(reinterpret_cast<char *>(b.host) + args_bytes);
-    int total = 0, lds_cells = 0;
-    int rc = fill_batch(ctx, jobs, njobs, dtype, args, offs, total, lds_cells);
+    int total = 0, lds_cells = 0, lds_q = 0;
+    int rc = fill_batch(ctx, jobs, njobs, dtype, args, offs, total, lds_cells, lds_q);
     if (rc) return rc;
     MCT_HIP(hipMemcpyAsync(b.dev, b.host, bytes, hipMemcpyHostToDevice, ctx->stream));
     const BpArgs *dargs = reinterpret_cast<const BpArgs *>(b.dev);
     const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(b.dev) + args_bytes);
-    rc = launch_batch(ctx, dargs, doffs, njobs, total, dtype, lds_cells);
+    rc = launch_batch(ctx, dargs, doffs, njobs, total, dtype, lds_cells, lds_q);
     const hipError_t e = hipEventRecord(b.ev, ctx->stream);  // after the kernel: slot reusable
     if (rc) return rc;
     if (e != hipSuccess) return mct::set_error(MCT_ERR_DEVICE, "batch staging event: %s", hipGetErrorString(e));
@@ -1647,8 +1784,8 @@ int mct_bp_batch_create(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs, mct
     const int dtype = jobs[0].dtype;
     std::vector<BpArgs> args((size_t)njobs);
     std::vector<int> offs((size_t)njobs + 1);
-    int total = 0, lds_cells = 0;
-    int rc = fill_batch(ctx, jobs, njobs, dtype, args.data(), offs.data(), total, lds_cells);
+    int total = 0, lds_cells = 0, lds_q = 0;
+    int rc = fill_batch(ctx, jobs, njobs, dtype, args.data(), offs.data(), total, lds_cells, lds_q);
     if (rc) return rc;
     const size_t args_bytes = sizeof(BpArgs) * (size_t)njobs;
     void *dev = nullptr;
@@ -1670,6 +1807,7 @@ int mct_bp_batch_create(mct_ctx *ctx, const mct_bp_job *jobs, int32_t njobs, mct
     b->total = total;
     b->dtype = dtype;
     b->lds_cells = lds_cells;
+    b->lds_q = lds_q;
     *out = b;
     return MCT_OK;
 }
@@ -1678,7 +1816,7 @@ int mct_bp_batch_launch(mct_bp_batch *b) {
     MCT_CHECK(b, "NULL batch");
     const BpArgs *dargs = reinterpret_cast<const BpArgs *>(b->dev);
     const int *doffs = reinterpret_cast<const int *>(reinterpret_cast<const char *>(b->dev) + b->args_bytes);
-    return launch_batch(b->ctx, dargs, doffs, b->njobs, b->total, b->dtype, b->lds_cells);
+    return launch_batch(b->ctx, dargs, doffs, b->njobs, b->total, b->dtype, b->lds_cells, b->lds_q);
 }
 
 int mct_bp_batch_destroy(mct_bp_batch *b) {

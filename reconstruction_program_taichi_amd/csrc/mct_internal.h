@@ -110,7 +110,7 @@ struct mct_bp_batch {
     int device = 0;
     void *dev = nullptr;  // BpArgs[njobs] followed by int offsets[njobs + 1]
     size_t args_bytes = 0;
-    int njobs = 0, total = 0, dtype = 0, lds_cells = 0;
+    int njobs = 0, total = 0, dtype = 0, lds_cells = 0, lds_q = 0;
 };
 
 namespace mct {

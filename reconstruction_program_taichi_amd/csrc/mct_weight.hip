@@ -152,6 +152,34 @@ __global__ void pack_pair_kernel(const float *__restrict__ in, int rows, int vie
     }
 }
 
+// ROWS: out[j][r + 1][c] = {in[r][j][c], in[r][j][c+1] - in[r][j][c]} (the
+// difference rounded once from FP64), r = -1 .. rows, rows -1 and `rows`
+// zero: a view's rows contiguous, each with the PAIR taps of its elements.
+template <int DT>
+__global__ void pack_rows_kernel(const float *__restrict__ in, int rows, int views, int nu,
+                                 void *__restrict__ out) {
+    const int nc = nu - 1;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, s = blockIdx.z;  // stored row s = r + 1
+    if (c >= nc) return;
+    const int r = s - 1;
+    float q0 = 0.f, q1 = 0.f;
+    if (r >= 0 && r < rows) {
+        const float *p = in + ((int64_t)r * views + j) * nu + c;
+        q0 = p[0];
+        q1 = (float)((double)p[1] - (double)p[0]);
+    }
+    const size_t idx = ((size_t)j * (rows + 2) + s) * nc + c;
+    if (DT == MCT_F32) {
+        reinterpret_cast<float2 *>(out)[idx] = make_float2(q0, q1);
+    } else {
+        ushort2 h;
+        h.x = to_half_bits<DT>(q0);
+        h.y = to_half_bits<DT>(q1);
+        reinterpret_cast<ushort2 *>(out)[idx] = h;
+    }
+}
+
 template <int DT>
 __global__ void pack_plain_kernel(const float *__restrict__ in, int64_t n, void *__restrict__ out) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,6 +248,9 @@ int64_t mct_packed_bytes(int32_t rows, int32_t views, int32_t n_u, int32_t dtype
         case MCT_PAIR:
             if (n_u < 2) return -1;
             return (int64_t)rows * views * (n_u - 1) * 2 * es;
+        case MCT_ROWS:
+            if (rows < 2 || n_u < 2) return -1;
+            return (int64_t)(rows + 2) * views * (n_u - 1) * 2 * es;
         default: return -1;
     }
 }
@@ -232,11 +263,14 @@ int mct_pack_sinogram(mct_ctx *ctx, const float *in, int32_t rows, int32_t views
     MCT_CHECK(bytes > 0, "bad packed shape (rows %d, views %d, n_u %d, layout %d)", rows, views,
               n_u, layout);
     MCT_CHECK(rows <= 65535 && views <= 65535, "rows and views must be <= 65535");
+    MCT_CHECK(layout != MCT_ROWS || rows <= 65533, "ROWS packing takes at most 65533 rows");
     dim3 grid;
     if (layout == MCT_QUAD)
         grid = dim3(mct::blocks_for(n_u - 1, 256), views, (rows + 1 + kQuadRows - 1) / kQuadRows);
     else if (layout == MCT_PAIR)
         grid = dim3(mct::blocks_for(n_u - 1, 256), rows, views);
+    else if (layout == MCT_ROWS)
+        grid = dim3(mct::blocks_for(n_u - 1, 256), views, rows + 2);
     int64_t n = (int64_t)rows * views * n_u;
 #define MCT_PACK_LAUNCH(K, G, ...)                                                            \
     do {                                                                                      \
@@ -251,6 +285,8 @@ int mct_pack_sinogram(mct_ctx *ctx, const float *in, int32_t rows, int32_t views
         MCT_PACK_LAUNCH(pack_quad_kernel, grid, in, rows, views, n_u, out);
     else if (layout == MCT_PAIR)
         MCT_PACK_LAUNCH(pack_pair_kernel, grid, in, rows, views, n_u, out);
+    else if (layout == MCT_ROWS)
+        MCT_PACK_LAUNCH(pack_rows_kernel, grid, in, rows, views, n_u, out);
     else
         MCT_PACK_LAUNCH(pack_plain_kernel, dim3(mct::blocks_for(n, 256)), in, n, out);
 #undef MCT_PACK_LAUNCH

@@ -17,7 +17,7 @@ from ._abi import call
 from .geometry import GeometrySpec
 
 DTYPES = {"f32": _abi.MCT_F32, "f16": _abi.MCT_F16, "bf16": _abi.MCT_BF16}
-LAYOUTS = {"plain": _abi.MCT_PLAIN, "quad": _abi.MCT_QUAD, "pair": _abi.MCT_PAIR}
+LAYOUTS = {"plain": _abi.MCT_PLAIN, "quad": _abi.MCT_QUAD, "pair": _abi.MCT_PAIR, "rows": _abi.MCT_ROWS}
 FILTER_METHODS = {"f32": _abi.MCT_FILTER_F32_DIRECT, "bf16": _abi.MCT_FILTER_BF16_MFMA,
                   "bf16x3": _abi.MCT_FILTER_BF16X3_MFMA}
 
@@ -324,7 +324,7 @@ class Plan:
         window over the LDS budget): then the direct kernel runs, about 1.8x
         slower.  Said once per plan."""
         s = self.spec
-        if (s.cone and layout == "quad" and self.ctx.bp_kernel == "auto"
+        if (s.cone and layout in ("quad", "rows") and self.ctx.bp_kernel == "auto"
                 and self.ctx.bp_last_kernel() == "direct"):
             self._warn_once("bp", "cone back-projection ran the direct-gather kernel: the LDS-staged "
                                   "kernel's tables do not fit this geometry (window table over 16 GiB "

@@ -491,9 +491,10 @@ class Mgfbp:
         return self._plan
 
     def tap_layout(self) -> str:
-        """Packed tap layout of the back-projector: QUAD (2 x 2 cells) for cone
-        beam, PAIR (neighbouring elements of a row) for fan beam."""
-        return "quad" if self.cone_beam else "pair"
+        """Packed tap layout of the back-projector: ROWS (each element with its
+        right neighbour's difference, rows padded by a zero row either side)
+        for cone beam, PAIR (the same, row-major over views) for fan beam."""
+        return "rows" if self.cone_beam else "pair"
 
     def weight_and_filter(self, sgm: torch.Tensor, view_begin: int | None = None) -> torch.Tensor:
         """BHCorrection + WeightSgm + FilterSinogram (run_mgfbp.py:63-70,

@@ -78,7 +78,7 @@ class Job:
         if self.kind == "cgls":
             return sg * 3 + vol * 7
         tap = 4 if self.storage == "f32" else 2
-        packed = (s.n_v + 1) * s.n_views * (s.n_u - 1) * 4 * tap
+        packed = (s.n_v + 2) * s.n_views * (s.n_u - 1) * 2 * tap
         return sg * 2 + packed + vol
 
 
@@ -130,7 +130,7 @@ def fbp_stages(ctx: Context, plan: Plan, job: Job, sgm: torch.Tensor):
         ctx.filter_rows(sgm, filt, job.taps, job.det_w, job.filter_method)
     else:
         plan.weight_filter(sgm, filt, job.taps, job.det_w, job.filter_method)
-    layout = "quad" if job.spec.cone else "pair"
+    layout = "rows" if job.spec.cone else "pair"
     return ctx.pack(filt, job.storage, layout), layout
 
 

@@ -49,7 +49,7 @@ def test_smallest_fbp_chain(ctx, kind, method, bp, nu):
         ctx.filter_rows(w, filt, taps, du, "f32")
     else:
         plan.weight_filter(w, filt, taps, du, method)
-    layout = "quad" if spec.cone else "pair"
+    layout = "rows" if spec.cone else "pair"
     vol = torch.full((spec.img_dim_z, spec.img_dim, spec.img_dim), 7.0, device="cuda")
     ctx.set_bp_kernel(bp)
     try:

@@ -201,15 +201,17 @@ def test_bp_fan_plain_generic(ctx):
     assert rel_l2(got, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("layout", ["quad", "rows"])
 @pytest.mark.parametrize("zc,batch", [(4, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 4), (32, 8),
                                       (32, 16)])
-def test_bp_cone_quad_tiles(ctx, zc, batch):
-    """Direct-gather kernel at every tile shape."""
+def test_bp_cone_quad_tiles(ctx, zc, batch, layout):
+    """Direct-gather kernel at every tile shape, from QUAD cells and from ROWS
+    element rows."""
     spec = cone_case(nz=37)  # taller than the detector's cone: top/bottom slices masked
     ctx.set_bp_tile(zc, batch)
     ctx.set_bp_kernel("direct")
     try:
-        got, ref, alive, _ = _bp_check(ctx, spec, "quad")
+        got, ref, alive, _ = _bp_check(ctx, spec, layout)
     finally:
         ctx.set_bp_tile(0, 0)
         ctx.set_bp_kernel("auto")
@@ -239,8 +241,9 @@ def _lds_spec(kind):
     raise ValueError(kind)
 
 
+@pytest.mark.parametrize("layout", ["quad", "rows"])
 @pytest.mark.parametrize("kind", ["tall", "top_row_offset", "odd_sizes", "rotated_offcentre", "curved"])
-def test_bp_cone_lds_staged(ctx, kind):
+def test_bp_cone_lds_staged(ctx, kind, layout):
     """LDS-staged kernel against the oracle (identical masks) and against the
     direct-gather kernel.  The two differ only in where the row coordinate is
     rounded (relative to the window's first row, or to detector row 0), about
@@ -248,14 +251,14 @@ def test_bp_cone_lds_staged(ctx, kind):
     spec = _lds_spec(kind)
     ctx.set_bp_kernel("lds")
     try:
-        got, ref, alive, f = _bp_check(ctx, spec, "quad", seed=21)
+        got, ref, alive, f = _bp_check(ctx, spec, layout, seed=21)
         assert ctx.bp_last_kernel() == "lds"
         ctx.set_bp_kernel("direct")
-        direct, _, _, _ = _bp_check(ctx, spec, "quad", seed=21)
+        direct, _, _, _ = _bp_check(ctx, spec, layout, seed=21)
         assert ctx.bp_last_kernel() == "direct"
     finally:
         ctx.set_bp_kernel("auto")
-    print(f"{kind}: lds vs oracle {rel_l2(got, ref):.2e}, direct vs oracle {rel_l2(direct, ref):.2e}, "
+    print(f"{kind} ({layout}): lds vs oracle {rel_l2(got, ref):.2e}, direct vs oracle {rel_l2(direct, ref):.2e}, "
           f"lds vs direct {rel_l2(got, direct):.2e}")
     assert mask_disagreements(got, ref, alive) == 0
     assert rel_l2(got, ref) <= 1e-5
@@ -263,20 +266,21 @@ def test_bp_cone_lds_staged(ctx, kind):
     assert rel_l2(got, direct) <= 4e-6
 
 
+@pytest.mark.parametrize("layout", ["quad", "rows"])
 @pytest.mark.parametrize("dtype", ["f16", "bf16"])
-def test_bp_cone_lds_half_storage_equals_direct(ctx, dtype):
+def test_bp_cone_lds_half_storage_equals_direct(ctx, dtype, layout):
     spec = cone_case(nz=37)
     rng = np.random.default_rng(13)
     f = (1.0 + 0.3 * rng.standard_normal((spec.n_v, spec.n_views, spec.n_u))).astype(np.float32)
     plan = Plan(ctx, spec)
-    packed = ctx.pack(dev(f), dtype, "quad")
+    packed = ctx.pack(dev(f), dtype, layout)
     out = {}
     for kind in ("lds", "direct"):
         ctx.set_bp_kernel(kind)
         try:
             vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32,
                               device=DEV)
-            plan.backproject_fbp(packed, dtype, "quad", vol)
+            plan.backproject_fbp(packed, dtype, layout, vol)
             out[kind] = host(vol)
         finally:
             ctx.set_bp_kernel("auto")
@@ -284,10 +288,11 @@ def test_bp_cone_lds_half_storage_equals_direct(ctx, dtype):
     assert rel_l2(out["lds"], out["direct"]) <= 1e-6
 
 
+@pytest.mark.parametrize("layout", ["quad", "rows"])
 @pytest.mark.parametrize("variant", ["top_row", "offset"])
-def test_bp_cone_variants(ctx, variant):
+def test_bp_cone_variants(ctx, variant, layout):
     spec = cone_case(top_row=True) if variant == "top_row" else cone_case(off_v=1.7, nz=30)
-    got, ref, alive, _ = _bp_check(ctx, spec, "quad", seed=5)
+    got, ref, alive, _ = _bp_check(ctx, spec, layout, seed=5)
     assert mask_disagreements(got, ref, alive) == 0
     assert rel_l2(got, ref) <= 1e-5
 
@@ -299,25 +304,27 @@ def test_bp_cone_plain_generic(ctx):
     assert rel_l2(got, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("layout", ["quad", "rows"])
 @pytest.mark.parametrize("dtype,tol", [("f16", 1e-3), ("bf16", 8e-3)])
-def test_bp_cone_half_storage(ctx, dtype, tol):
+def test_bp_cone_half_storage(ctx, dtype, tol, layout):
     spec = cone_case()
     rng = np.random.default_rng(9)
     # smooth positive data, as a filtered sinogram of a phantom is at this scale
     f = (1.0 + 0.3 * rng.standard_normal((spec.n_v, spec.n_views, spec.n_u))).astype(np.float32)
     plan = Plan(ctx, spec)
-    packed = ctx.pack(dev(f), dtype, "quad")
+    packed = ctx.pack(dev(f), dtype, layout)
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device=DEV)
-    plan.backproject_fbp(packed, dtype, "quad", vol)
+    plan.backproject_fbp(packed, dtype, layout, vol)
     got = host(vol)
     ref, alive = recon.backproject_fbp(f, oracle_dict(spec), np.float64, return_mask=True)
     keep = alive & (got != 0)
     err = rel_l2(got[keep], ref[keep])
-    print(f"{dtype} storage: rel-L2 vs FP64 oracle {err:.3e}")
+    print(f"{dtype} storage ({layout}): rel-L2 vs FP64 oracle {err:.3e}")
     assert err <= tol
 
 
-def test_fdk_pipeline_small(ctx):
+@pytest.mark.parametrize("layout", ["quad", "rows"])
+def test_fdk_pipeline_small(ctx, layout):
     """weight -> ramp filter -> pack -> back-project against the oracle chain."""
     spec = cone_case(nu=80, nrows=32)
     rng = np.random.default_rng(11)
@@ -328,9 +335,9 @@ def test_fdk_pipeline_small(ctx):
     plan.weight(t)
     filt = torch.empty_like(t)
     ctx.filter_rows(t, filt, taps, 0.6, "f32")
-    packed = ctx.pack(filt, "f32", "quad")
+    packed = ctx.pack(filt, "f32", layout)
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device=DEV)
-    plan.backproject_fbp(packed, "f32", "quad", vol)
+    plan.backproject_fbp(packed, "f32", layout, vol)
     got = host(vol)
     w = recon.weight_sinogram(sgm, spec.u, spec.v, spec.angles, spec.sdd, spec.total_scan_angle,
                               False, dtype=np.float32)

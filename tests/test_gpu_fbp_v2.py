@@ -169,10 +169,10 @@ def test_backproject_views_rejects_ranges_outside_the_scan(tmp_path):
     plan, ctx = f.plan(), f.ctx()
     vol = torch.zeros((24, 48, 48), device="cuda")
     blk = torch.zeros((40, 10, 96), device="cuda")
-    packed = ctx.pack(blk, "f32", "quad")
+    packed = ctx.pack(blk, "f32", "rows")
     # the binding refuses the range before any call ...
     with pytest.raises(ValueError, match="outside the plan"):
-        plan.backproject_views(packed, "f32", "quad", 85, 10, vol)
+        plan.backproject_views(packed, "f32", "rows", 85, 10, vol)
     with pytest.raises(ValueError, match="outside the plan"):
         plan.weight_views(blk, -1)
     # ... and so does the C ABI itself
@@ -180,7 +180,7 @@ def test_backproject_views_rejects_ranges_outside_the_scan(tmp_path):
     from reconstruction_program_taichi_amd import _abi
     from reconstruction_program_taichi_amd.device import DTYPES, LAYOUTS
     with pytest.raises(RuntimeError, match="outside the scan"):
-        _abi.call("mct_backproject_views", plan.h, packed.data_ptr(), DTYPES["f32"], LAYOUTS["quad"],
+        _abi.call("mct_backproject_views", plan.h, packed.data_ptr(), DTYPES["f32"], LAYOUTS["rows"],
                   85, 10, 0, vol.data_ptr())
     with pytest.raises(RuntimeError, match="outside the scan"):
         _abi.call("mct_weight_views", plan.h, blk.data_ptr(), -1, 10, None, 0)

@@ -35,7 +35,7 @@ def test_batched_backprojection_graph_replays(ctx):
     rng = np.random.default_rng(5)
     jobs = []
     for s in specs:
-        layout = "quad" if s.cone else "pair"
+        layout = "rows" if s.cone else "pair"
         f = torch.from_numpy(rng.standard_normal((s.n_v, s.n_views, s.n_u)).astype(np.float32)).cuda()
         plan = Plan(ctx, s)
         packed = ctx.pack(f, "f32", layout)
@@ -74,15 +74,15 @@ def test_fdk_pipeline_graph_replays(ctx, method):
     plan = Plan(ctx, spec)
     work = torch.empty_like(raw)
     filt = torch.empty_like(raw)
-    nbytes = ctx.pack(raw, "f32", "quad").numel()
+    nbytes = ctx.pack(raw, "f32", "rows").numel()
     packed = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
 
     def pipeline():
         plan.weight(work)
         ctx.filter_rows(work, filt, taps, 0.6, method)
-        ctx.pack_into(filt, "f32", "quad", packed)
-        plan.backproject_fbp(packed, "f32", "quad", vol)
+        ctx.pack_into(filt, "f32", "rows", packed)
+        plan.backproject_fbp(packed, "f32", "rows", vol)
 
     work.copy_(raw)
     pipeline()  # eager: builds the tap caches and the back-projector's tables
@@ -127,14 +127,14 @@ def test_frozen_context_keeps_a_captured_pipeline_valid(method):
     taps = recon.ramlak_taps(spec.n_u, 0.6, np.float32)
     plan = Plan(ctx, spec)
     work, filt = torch.empty_like(raw), torch.empty_like(raw)
-    packed = torch.empty(ctx.pack(raw, "f32", "quad").numel(), dtype=torch.uint8, device="cuda")
+    packed = torch.empty(ctx.pack(raw, "f32", "rows").numel(), dtype=torch.uint8, device="cuda")
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
 
     def pipeline():
         plan.weight(work)
         ctx.filter_rows(work, filt, taps, 0.6, method)
-        ctx.pack_into(filt, "f32", "quad", packed)
-        plan.backproject_fbp(packed, "f32", "quad", vol)
+        ctx.pack_into(filt, "f32", "rows", packed)
+        plan.backproject_fbp(packed, "f32", "rows", vol)
 
     work.copy_(raw)
     pipeline()

@@ -3,7 +3,7 @@ of a 1024 x 1024 detector at 0.3 mm, SID 750 / SDD 1250 mm).
 
 - Back-projection: the LDS-staged kernel's window table is 2.3 GB here
   (64 x 64 tiles x 64 slice chunks x 1080 views x 8 B) and the packed
-  sinogram 18 GB; the kernel runs (no fallback) and two slices (the first
+  sinogram (ROWS layout) 9 GB; the kernel runs (no fallback) and two slices (the first
   and a middle one) match the oracle's C restatement, masks identical.
   The input is a smooth field, as filtered projections are: the kernel's
   magnification comes from the hardware reciprocal (about 1 ulp), which
@@ -48,11 +48,11 @@ def test_backprojection_of_a_1024_cube_with_a_2gb_window_table(ctx):
     filt = torch.sin(c * (2 * np.pi / 173) + ph[0]) * torch.cos(r * (2 * np.pi / 311) + ph[1])
     filt += 0.5 * torch.sin(c * (2 * np.pi / 251) + ph[2]) * torch.cos(r * (2 * np.pi / 197) + ph[3])
     plan = Plan(ctx, spec)
-    packed = ctx.pack(filt, "f32", "quad")
+    packed = ctx.pack(filt, "f32", "rows")
     vol = torch.empty((1024, 1024, 1024), dtype=torch.float32, device="cuda")
     with warnings.catch_warnings():
         warnings.simplefilter("error", RuntimeWarning)  # no fallback
-        plan.backproject_fbp(packed, "f32", "quad", vol)
+        plan.backproject_fbp(packed, "f32", "rows", vol)
     assert ctx.bp_last_kernel() == "lds"
     del packed
     fh = filt.cpu().numpy()

@@ -52,7 +52,7 @@ def _spec(rot=0.0, tilt=False):
 
 
 @pytest.mark.parametrize("case", ["plain", "rot", "tilt"])
-@pytest.mark.parametrize("layout", ["quad", "plain"])
+@pytest.mark.parametrize("layout", ["quad", "rows", "plain"])
 def test_bp_pmatrix_matches_oracle(ctx, case, layout):
     spec = _spec(rot=0.375 if case == "rot" else 0.0, tilt=case == "tilt")
     rng = np.random.default_rng(21)

@@ -16,8 +16,8 @@ from tests._cases import cone_case, fan_case, oracle_dict, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _single(ctx, spec, f, dtype):
-    layout = "quad" if spec.cone else "pair"
+def _single(ctx, spec, f, dtype, cone_layout="rows"):
+    layout = cone_layout if spec.cone else "pair"
     plan = Plan(ctx, spec)
     packed = ctx.pack(f, dtype, layout)
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
@@ -25,9 +25,10 @@ def _single(ctx, spec, f, dtype):
     return plan, packed, layout, vol
 
 
+@pytest.mark.parametrize("cone_layout", ["rows", "quad"])
 @pytest.mark.parametrize("kernel", ["auto", "direct"])
 @pytest.mark.parametrize("dtype", ["f32", "f16"])
-def test_batched_bp_equals_single_jobs(ctx, dtype, kernel):
+def test_batched_bp_equals_single_jobs(ctx, dtype, kernel, cone_layout):
     """The cone jobs run the LDS-staged tile (auto) or the direct one, in the
     batch and alone alike; with FP32 taps every batched volume also matches
     the FP32 reference-order oracle (rel-L2 <= 1e-5, the same voxels masked)."""
@@ -40,7 +41,7 @@ def test_batched_bp_equals_single_jobs(ctx, dtype, kernel):
         for s in specs:
             fhost.append(rng.standard_normal((s.n_v, s.n_views, s.n_u)).astype(np.float32))
             f = torch.from_numpy(fhost[-1]).cuda()
-            plan, packed, layout, vol = _single(ctx, s, f, dtype)
+            plan, packed, layout, vol = _single(ctx, s, f, dtype, cone_layout)
             if s.cone:
                 assert ctx.bp_last_kernel() == ("lds" if kernel == "auto" else "direct")
             singles.append(vol.clone())

@@ -123,9 +123,9 @@ def test_c1_weight_filter_and_backprojection_match_oracle(c1):
     taps = fbp.recon_kernel()
     assert fbp.kernel_name == "SheppLoganFilter"
     ctx.filter_rows(w, filt, taps, 0.4, "f32")
-    packed = ctx.pack(filt, "f32", "quad")
+    packed = ctx.pack(filt, "f32", "rows")
     vol = torch.empty((256, 512, 512), dtype=torch.float32, device="cuda")
-    plan.backproject_fbp(packed, "f32", "quad", vol)
+    plan.backproject_fbp(packed, "f32", "rows", vol)
     assert ctx.bp_last_kernel() == "lds"
     got = vol.cpu().numpy()
     f_host = filt.cpu().numpy()

@@ -35,6 +35,10 @@ def test_packed_bytes_host_helper():
     lib = _abi.load()
     # QUAD: cell rows -1 .. rows - 1 (the detector's 255 and one zero-padded row either side)
     assert lib.mct_packed_bytes(256, 1080, 1024, _abi.MCT_F32, _abi.MCT_QUAD) == 257 * 1080 * 1023 * 16
+    # ROWS: element rows -1 .. rows (one zero-padded row either side), PAIR taps
+    assert lib.mct_packed_bytes(256, 1080, 1024, _abi.MCT_F32, _abi.MCT_ROWS) == 258 * 1080 * 1023 * 8
+    assert lib.mct_packed_bytes(256, 1080, 1024, _abi.MCT_BF16, _abi.MCT_ROWS) == 258 * 1080 * 1023 * 4
+    assert lib.mct_packed_bytes(1, 10, 8, _abi.MCT_F32, _abi.MCT_ROWS) == -1
     assert lib.mct_packed_bytes(1, 720, 800, _abi.MCT_F16, _abi.MCT_PAIR) == 720 * 799 * 4
     assert lib.mct_packed_bytes(3, 10, 8, _abi.MCT_BF16, _abi.MCT_PLAIN) == 3 * 10 * 8 * 2
     assert lib.mct_packed_bytes(1, 10, 8, _abi.MCT_F32, _abi.MCT_QUAD) == -1
