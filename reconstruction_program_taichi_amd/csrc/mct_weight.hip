@@ -155,6 +155,9 @@ __global__ void pack_pair_kernel(const float *__restrict__ in, int rows, int vie
 // ROWS: out[j][r + 1][c] = {in[r][j][c], in[r][j][c+1] - in[r][j][c]} (the
 // difference rounded once from FP64), r = -1 .. rows, rows -1 and `rows`
 // zero: a view's rows contiguous, each with the PAIR taps of its elements.
+// One element per thread: every load and store of a wavefront is contiguous
+// (4 elements per thread with 16-byte loads measured 1.12 ms against 0.89
+// ms at 256 x 1080 x 1024, its 8-byte stores strided 32 bytes apart).
 template <int DT>
 __global__ void pack_rows_kernel(const float *__restrict__ in, int rows, int views, int nu,
                                  void *__restrict__ out) {
