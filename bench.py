@@ -409,7 +409,9 @@ def main() -> int:
         line["config"]["pcie_inclusive"] = pcie_inclusive(raw_h, work, vol, fbp)
         line["config"]["streamed"] = streamed_inclusive(raw_h, vol, args.storage, args.filter)
     if not args.no_cpu:
-        filt_host = fbp.last_filtered.cpu().numpy()
+        # the FP32 filtered sinogram of the step (the headline step writes the
+        # back-projector's taps straight from the filter, the same values)
+        filt_host = fbp.weight_and_filter(raw.clone()).cpu().numpy()
         cb = cpu_baseline(fbp, filt_host, raw[:16].cpu().numpy(), vol.cpu().numpy(), args.cpu_slices)
         check = cb.pop("_check")
         z0 = nz // 2 - args.cpu_slices // 2

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MCT_ABI_VERSION 10
+#define MCT_ABI_VERSION 11
 
 typedef enum mct_status {
     MCT_OK = 0,
@@ -183,6 +183,17 @@ int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, i
  * method: MCT_FILTER_BF16_MFMA or MCT_FILTER_BF16X3_MFMA only.  There is no
  * vertical Gaussian between the two steps here. */
 int mct_weight_filter_rows(mct_plan *plan, const float *sgm, float *out, int32_t view_begin,
+                           int32_t view_count, const float *bh_coef, int32_t bh_order,
+                           const float *taps, float scale, int32_t method);
+
+/* mct_weight_filter_rows followed by mct_pack_sinogram(..., MCT_F32,
+ * MCT_ROWS), bit for bit, without the FP32 filtered sinogram: the filter's
+ * output is written straight as the back-projector's ROWS taps
+ * (run_mgfbp.py:946-1046 and the filtered field img_sgm_filtered_taichi,
+ * :1230-1242).  packed DEVICE, mct_packed_bytes(n_v, view_count, n_u,
+ * MCT_F32, MCT_ROWS) bytes; needs n_v >= 2 and n_u - 1 not a multiple of 64
+ * (else filter and pack separately). */
+int mct_weight_filter_pack(mct_plan *plan, const float *sgm, void *packed, int32_t view_begin,
                            int32_t view_count, const float *bh_coef, int32_t bh_order,
                            const float *taps, float scale, int32_t method);
 

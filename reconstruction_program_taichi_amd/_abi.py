@@ -24,7 +24,7 @@ MCT_FILTER_F32_DIRECT, MCT_FILTER_BF16_MFMA, MCT_FILTER_BF16X3_MFMA = 0, 1, 2
 MCT_BP_AUTO, MCT_BP_DIRECT, MCT_BP_LDS = 0, 1, 2
 MCT_FP_AUTO, MCT_FP_DIRECT, MCT_FP_CELLS = 0, 1, 2
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 
 class MctError(RuntimeError):
@@ -90,6 +90,8 @@ SIGNATURES = {
     "mct_filter_vertical": (C.c_int, [_P, _P, _P, _I32, _I32, _I32, C.POINTER(_F32), _F32]),
     "mct_filter_rows": (C.c_int, [_P, _P, _P, _I64, _I32, C.POINTER(_F32), _F32, _I32]),
     "mct_weight_filter_rows": (C.c_int, [_P, _P, _P, _I32, _I32, C.POINTER(_F32), _I32,
+                                         C.POINTER(_F32), _F32, _I32]),
+    "mct_weight_filter_pack": (C.c_int, [_P, _P, _P, _I32, _I32, C.POINTER(_F32), _I32,
                                          C.POINTER(_F32), _F32, _I32]),
     "mct_pack_sinogram": (C.c_int, [_P, _P, _I32, _I32, _I32, _P, _I32, _I32]),
     "mct_packed_bytes": (_I64, [_I32, _I32, _I32, _I32, _I32]),

@@ -78,8 +78,40 @@ __global__ __launch_bounds__(256) void ramp_direct_kernel(const float *__restric
 
 namespace mct {
 int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, int n,
-                     const float *taps, float scale, int method, const WeighSpec *w);
+                     const float *taps, float scale, int method, const WeighSpec *w, int rows_views);
 }
+
+namespace {
+
+// Checks the arguments of a fused weighting + MFMA filter call and fills the
+// weighting of views [view_begin, view_begin + view_count) of the plan.
+int weigh_spec(mct_plan *plan, int32_t view_begin, int32_t view_count, const float *bh_coef,
+               int32_t bh_order, int32_t method, mct::WeighSpec &w) {
+    MCT_CHECK(method == MCT_FILTER_BF16_MFMA || method == MCT_FILTER_BF16X3_MFMA,
+              "weighting is fused only into the MFMA filter methods (got %d)", method);
+    MCT_CHECK(bh_order >= 0 && bh_order <= mct::kMaxBH, "bh_order must be in [0, %d]", mct::kMaxBH);
+    MCT_CHECK(bh_order == 0 || bh_coef, "bh_coef is NULL");
+    const mct_geometry &g = plan->g;
+    MCT_CHECK(view_begin >= 0 && view_count >= 1 && view_count <= g.n_views - view_begin,
+              "views [%d, %d + %d) outside the scan's %d views", view_begin, view_begin, view_count,
+              g.n_views);
+    MCT_CHECK(g.n_u >= 1 && g.n_u <= 4096, "row length must be in [1, 4096] (got %d)", g.n_u);
+    w = {};
+    w.u = plan->u;
+    w.v = plan->v;
+    w.ang = plan->angles;
+    w.sdd = g.sdd;
+    w.scan = g.total_scan_angle;
+    w.short_scan = g.short_scan;
+    w.curved = g.curved;
+    for (int t = 0; t < bh_order; ++t) w.bh.c[t] = bh_coef[t];
+    w.bh_order = bh_order;
+    w.nviews = view_count;
+    w.view0 = view_begin;
+    return MCT_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -92,7 +124,7 @@ int mct_filter_rows(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, i
     MCT_CHECK(in && out, "NULL argument");
     MCT_CHECK(in != out, "the row filter cannot run in place");
     if (method != MCT_FILTER_F32_DIRECT)
-        return mct::filter_rows_mfma(ctx, in, out, n_rows, n, taps, scale, method, nullptr);
+        return mct::filter_rows_mfma(ctx, in, out, n_rows, n, taps, scale, method, nullptr, 0);
     int rc = mct::upload_cached(ctx, &ctx->taps_dev, &ctx->taps_cap, &ctx->taps_hash, taps,
                                 2 * (int64_t)n - 1);
     if (rc) return rc;
@@ -113,29 +145,28 @@ int mct_weight_filter_rows(mct_plan *plan, const float *sgm, float *out, int32_t
                            const float *taps, float scale, int32_t method) {
     MCT_CHECK(plan && sgm && out && taps, "NULL argument");
     MCT_CHECK(sgm != out, "the row filter cannot run in place");
-    MCT_CHECK(method == MCT_FILTER_BF16_MFMA || method == MCT_FILTER_BF16X3_MFMA,
-              "weighting is fused only into the MFMA filter methods (got %d)", method);
-    MCT_CHECK(bh_order >= 0 && bh_order <= mct::kMaxBH, "bh_order must be in [0, %d]", mct::kMaxBH);
-    MCT_CHECK(bh_order == 0 || bh_coef, "bh_coef is NULL");
+    mct::WeighSpec w;
+    int rc = weigh_spec(plan, view_begin, view_count, bh_coef, bh_order, method, w);
+    if (rc) return rc;
     const mct_geometry &g = plan->g;
-    MCT_CHECK(view_begin >= 0 && view_count >= 1 && view_count <= g.n_views - view_begin,
-              "views [%d, %d + %d) outside the scan's %d views", view_begin, view_begin, view_count,
-              g.n_views);
-    MCT_CHECK(g.n_u >= 1 && g.n_u <= 4096, "row length must be in [1, 4096] (got %d)", g.n_u);
-    mct::WeighSpec w = {};
-    w.u = plan->u;
-    w.v = plan->v;
-    w.ang = plan->angles;
-    w.sdd = g.sdd;
-    w.scan = g.total_scan_angle;
-    w.short_scan = g.short_scan;
-    w.curved = g.curved;
-    for (int t = 0; t < bh_order; ++t) w.bh.c[t] = bh_coef[t];
-    w.bh_order = bh_order;
-    w.nviews = view_count;
-    w.view0 = view_begin;
     return mct::filter_rows_mfma(plan->ctx, sgm, out, (int64_t)g.n_v * view_count, g.n_u, taps, scale,
-                                 method, &w);
+                                 method, &w, 0);
+}
+
+int mct_weight_filter_pack(mct_plan *plan, const float *sgm, void *packed, int32_t view_begin,
+                           int32_t view_count, const float *bh_coef, int32_t bh_order,
+                           const float *taps, float scale, int32_t method) {
+    MCT_CHECK(plan && sgm && packed && taps, "NULL argument");
+    mct::WeighSpec w;
+    int rc = weigh_spec(plan, view_begin, view_count, bh_coef, bh_order, method, w);
+    if (rc) return rc;
+    const mct_geometry &g = plan->g;
+    MCT_CHECK(g.n_v >= 2 && g.n_u >= 2 && (g.n_u - 1) % 64 != 0,
+              "fused ROWS output needs at least 2 rows and n_u - 1 not a multiple of 64 "
+              "(rows %d, n_u %d): filter, then mct_pack_sinogram", g.n_v, g.n_u);
+    MCT_CHECK(g.n_v <= 65533, "ROWS packing takes at most 65533 rows");
+    return mct::filter_rows_mfma(plan->ctx, sgm, reinterpret_cast<float *>(packed),
+                                 (int64_t)g.n_v * view_count, g.n_u, taps, scale, method, &w, view_count);
 }
 
 }  // extern "C"

@@ -119,13 +119,27 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict
     if (SPLIT) *reinterpret_cast<bf16x8 *>(lo + r * npad + m0) = l;
 }
 
-template <bool SPLIT>
+// Lane l's value from lane (l & ~15) | ((l + 1) & 15): the next column of a
+// 16 x 16 accumulator block (DPP row rotate by 15 within each row of 16).
+__device__ __forceinline__ float next_col(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x12F, 0xF, 0xF, false));
+}
+
+// ROWS_OUT: instead of the FP32 rows, the epilogue writes the back-projector's
+// ROWS taps of every filtered element, {y[c], y[c+1] - y[c]} (FP32), to
+// out[view][row + 1][c] (row = r / views, view = r % views of GEMM row r;
+// out has det_rows + 2 rows of n - 1 entries per view).  A column's right
+// neighbour is the next lane of its accumulator block, or the first of the
+// next block; the last column of each wavefront's 64 has its neighbour in
+// another wavefront, and rows_fixup_kernel completes its difference.
+template <bool SPLIT, bool ROWS_OUT>
 __global__ __launch_bounds__(256) void ramp_gemm_kernel(const __bf16 *__restrict__ plane_hi,
                                                         const __bf16 *__restrict__ plane_lo,
                                                         float *__restrict__ out, int64_t rows, int n,
                                                         int npad, int col_tiles, int nxcd,
                                                         const __bf16 *__restrict__ taps_hi,
-                                                        const __bf16 *__restrict__ taps_lo, float scale) {
+                                                        const __bf16 *__restrict__ taps_lo, float scale,
+                                                        int views, int det_rows) {
     __shared__ __attribute__((aligned(16))) __bf16 a_hi[2][TM * APAD];
     __shared__ __attribute__((aligned(16))) __bf16 a_lo[2][SPLIT ? TM * APAD : 8];
     __shared__ __attribute__((aligned(16))) __bf16 w_hi[8 * WLC];
@@ -223,18 +237,68 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(const __bf16 *__restrict
             }
         if (step + 1 < nsteps) store_a(cur ^ 1);  // no wave reads buffer cur^1 after this step's barrier
     }
+    if constexpr (ROWS_OUT) {
+        float2 *po = reinterpret_cast<float2 *>(out);
+        const int nc = n - 1;
+        const bool last_lane = (lane & 15) == 15;
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const int col = k0 + wc * 64 + cb * 16 + (lane & 15);
-            if (col >= n) continue;
+        for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t r = r0 + wr * 64 + rb * 16 + (lane >> 4) * 4 + i;
-                if (r < rows) out[r * n + col] = acc[rb][cb][i] * scale;
+                const int64_t dr = r / views, vw = r - dr * views;
+                const int64_t base = (vw * (det_rows + 2) + dr + 1) * nc;
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    const float y = acc[rb][cb][i] * scale;
+                    const float ys = next_col(y);
+                    const float yn = cb < 3 ? next_col(acc[rb][cb < 3 ? cb + 1 : cb][i] * scale) : 0.f;
+                    const float nb = last_lane ? yn : ys;
+                    const int col = k0 + wc * 64 + cb * 16 + (lane & 15);
+                    if (r < rows && col < nc) po[base + col] = make_float2(y, nb - y);
+                }
             }
-        }
+    } else {
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const int col = k0 + wc * 64 + cb * 16 + (lane & 15);
+                if (col >= n) continue;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t r = r0 + wr * 64 + rb * 16 + (lane >> 4) * 4 + i;
+                    if (r < rows) out[r * n + col] = acc[rb][cb][i] * scale;
+                }
+            }
+    }
+}
+
+// Completes the ROWS taps written by ramp_gemm_kernel<., true>: the
+// difference of every column 64 t + 63 < n - 1 (its neighbour was in another
+// wavefront; both values are final now), and the zero rows 0 and det_rows + 1
+// of every view.
+__global__ __launch_bounds__(256) void rows_fixup_kernel(float2 *__restrict__ po, int64_t rows, int n,
+                                                         int views, int det_rows) {
+    const int nc = n - 1;
+    // elements c = 64 t + 63 <= nc - 1 (their neighbour c + 1 <= nc - 1 is an
+    // element too: the caller keeps nc from being a multiple of 64)
+    const int nb = nc >= 64 ? (nc - 64) / 64 + 1 : 0;
+    const int64_t nfix = rows * nb, nzero = (int64_t)views * 2 * nc;
+    const int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (id < nfix) {
+        const int64_t r = id / nb;
+        const int c = (int)(id - r * nb) * 64 + 63;
+        const int64_t dr = r / views, vw = r - dr * views;
+        float2 *e = po + (vw * (det_rows + 2) + dr + 1) * nc + c;
+        e[0].y = e[1].x - e[0].x;
+    } else if (id < nfix + nzero) {
+        const int64_t k = id - nfix;
+        const int64_t vw = k / (2 * nc);
+        const int rest = (int)(k - vw * 2 * nc);
+        const int s = rest < nc ? 0 : det_rows + 1, c = rest < nc ? rest : rest - nc;
+        po[(vw * (det_rows + 2) + s) * nc + c] = make_float2(0.f, 0.f);
+    }
 }
 
 uint16_t bf16_bits_rn(float f) {
@@ -258,9 +322,12 @@ using mct::fnv1a;
 namespace mct {
 
 // w != NULL: the input rows are an unweighted [n_v][w->nviews][n] block,
-// weighed in the split pass (n_rows = n_v * w->nviews).
+// weighed in the split pass (n_rows = n_v * w->nviews).  rows_views > 0:
+// out receives the FP32 ROWS taps of the [n_rows / rows_views][rows_views][n]
+// rows instead (views major, rows padded by a zero row either side; n - 1
+// must not be a multiple of 64).
 int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, int n,
-                     const float *taps, float scale, int method, const WeighSpec *w) {
+                     const float *taps, float scale, int method, const WeighSpec *w, int rows_views) {
     MCT_CHECK(method == MCT_FILTER_BF16_MFMA || method == MCT_FILTER_BF16X3_MFMA,
               "unknown filter method %d", method);
     const int ntaps = 2 * n;  // running sums G[0 .. 2n-1] of the 2n-1 taps
@@ -322,6 +389,22 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
     const dim3 sgrid((unsigned)((groups + 255) / 256));
     const dim3 grid((unsigned)(row_tiles * col_tiles));
     const WeighSpec wv = w ? *w : WeighSpec{};
+    const int rv = rows_views > 0 ? rows_views : 1;
+    const int det_rows = (int)(n_rows / rv);
+    if (rows_views > 0)
+        MCT_CHECK(n >= 2 && (n - 1) % 64 != 0 && n_rows % rows_views == 0,
+                  "ROWS output needs n - 1 not a multiple of 64 (n = %d)", n);
+#define GEMM(SPLIT)                                                                              \
+    do {                                                                                         \
+        if (rows_views > 0)                                                                      \
+            hipLaunchKernelGGL((ramp_gemm_kernel<SPLIT, true>), grid, dim3(256), 0, ctx->stream, \
+                               phi, plo, out, n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, \
+                               scale, rv, det_rows);                                             \
+        else                                                                                     \
+            hipLaunchKernelGGL((ramp_gemm_kernel<SPLIT, false>), grid, dim3(256), 0, ctx->stream, \
+                               phi, plo, out, n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, \
+                               scale, rv, det_rows);                                             \
+    } while (0)
     if (split) {
         if (w)
             hipLaunchKernelGGL((split_bf16_kernel<true, true>), sgrid, dim3(256), 0, ctx->stream, in,
@@ -329,8 +412,7 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
         else
             hipLaunchKernelGGL((split_bf16_kernel<true, false>), sgrid, dim3(256), 0, ctx->stream, in,
                                n_rows, n, npad, phi, plo, wv);
-        hipLaunchKernelGGL(ramp_gemm_kernel<true>, grid, dim3(256), 0, ctx->stream, phi, plo, out,
-                           n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, scale);
+        GEMM(true);
     } else {
         if (w)
             hipLaunchKernelGGL((split_bf16_kernel<false, true>), sgrid, dim3(256), 0, ctx->stream, in,
@@ -338,8 +420,14 @@ int filter_rows_mfma(mct_ctx *ctx, const float *in, float *out, int64_t n_rows, 
         else
             hipLaunchKernelGGL((split_bf16_kernel<false, false>), sgrid, dim3(256), 0, ctx->stream, in,
                                n_rows, n, npad, phi, plo, wv);
-        hipLaunchKernelGGL(ramp_gemm_kernel<false>, grid, dim3(256), 0, ctx->stream, phi, plo, out,
-                           n_rows, n, npad, col_tiles, ctx->xcd_count, thi, tlo, scale);
+        GEMM(false);
+    }
+#undef GEMM
+    if (rows_views > 0) {
+        const int nc = n - 1, nb = nc >= 64 ? (nc - 64) / 64 + 1 : 0;
+        const int64_t work = n_rows * nb + (int64_t)rv * 2 * nc;
+        hipLaunchKernelGGL(rows_fixup_kernel, dim3(mct::blocks_for(work, 256)), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<float2 *>(out), n_rows, n, rv, det_rows);
     }
     MCT_LAUNCHED();
     return MCT_OK;

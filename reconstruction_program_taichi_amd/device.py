@@ -413,6 +413,28 @@ class Plan:
                  len(keep), tp, float(scale), FILTER_METHODS[method])
         return out
 
+    def weight_filter_pack(self, sgm: torch.Tensor, packed: torch.Tensor, taps: np.ndarray, scale: float,
+                           method: str, view_begin: int = 0, bh_coef=None) -> torch.Tensor:
+        """weight_filter(sgm, ...) then the context's pack(..., "f32", "rows")
+        into packed, in one pass and without the FP32 filtered sinogram (the
+        filter writes the ROWS taps itself); MFMA methods only, n_u - 1 not a
+        multiple of 64."""
+        rows, n, nu = sgm.shape
+        self._check_sgm(sgm, views=n)
+        self._check_views(view_begin, n)
+        self._check_packed(packed, "f32", "rows", n)
+        if taps.shape[0] != 2 * nu - 1:
+            raise ValueError(f"expected {2 * nu - 1} taps, got {taps.shape[0]}")
+        keep_t, tp = _host_f32(taps)
+        if bh_coef is None or len(bh_coef) == 0:
+            call("mct_weight_filter_pack", self.h, _f32(sgm), _ptr(packed), int(view_begin), n, None, 0,
+                 tp, float(scale), FILTER_METHODS[method])
+        else:
+            keep, cp = _host_f32(bh_coef)
+            call("mct_weight_filter_pack", self.h, _f32(sgm), _ptr(packed), int(view_begin), n, cp,
+                 len(keep), tp, float(scale), FILTER_METHODS[method])
+        return packed
+
     def backproject_views(self, filt: torch.Tensor, dtype: str, layout: str, view_begin: int,
                           view_count: int, vol: torch.Tensor, finish: bool = False) -> torch.Tensor:
         """Adds the back-projection of views [view_begin, view_begin +

@@ -33,7 +33,7 @@ from . import geometry as geo
 from . import pmatrix as pmx
 from .config import (PI, ReadConfigFile, fail, first_key, imreadRaw, imwriteRaw, imwriteTiff,
                      is_num, save_jsonc)
-from .device import Context, Plan
+from .device import Context, Plan, _packed_bytes
 
 
 def run_mgfbp(file_path):
@@ -528,16 +528,39 @@ class Mgfbp:
         ctx.filter_rows(src, filt, taps, self.det_elem_width, self.filter_method)
         return filt
 
+    def fused_filter_pack(self) -> bool:
+        """Whether weighting, MFMA ramp filter and the FP32 ROWS repack run as
+        one pass (mct_weight_filter_pack): cone beam, FP32 storage, an MFMA
+        filter method, no vertical Gaussian, no filtered sinogram to save, and
+        a row length the fused epilogue takes."""
+        return (self.cone_beam and self.storage == "f32" and self.filter_method != "f32"
+                and getattr(self, "kernel_name", None) in ("HammingFilter", "GaussianApodizedRamp",
+                                                           "SheppLoganFilter")
+                and not self.apply_gauss_vertical
+                and not self.save_filtered_sinogram and self.tap_layout() == "rows"
+                and self.det_elem_count_vertical_actual >= 2
+                and (self.det_elem_count_horizontal - 1) % 64 != 0)
+
     def reconstruct(self, sgm: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """The device pipeline of MainFunction (run_mgfbp.py:63-81) on a
         resident FP32 sinogram (rows, views, n_u); sgm is weighted in place
         (as the reference mutates its ndarray argument) unless the weighting
         runs inside the ramp filter's input pass (weight_and_filter)."""
         ctx, plan = self.ctx(), self.plan()
-        filt = self.weight_and_filter(sgm)
-        self.last_filtered = filt
         layout = self.tap_layout()
-        packed = ctx.pack(filt, self.storage, layout)
+        if self.fused_filter_pack():
+            # the MFMA filter writes the back-projector's ROWS taps itself
+            # (no FP32 filtered sinogram; the same bits as filter then pack)
+            nbytes = _packed_bytes(sgm.shape[0], sgm.shape[1], sgm.shape[2], "f32", "rows")
+            packed = torch.empty(nbytes, dtype=torch.uint8, device=sgm.device)
+            bh = self.array_bh_coefficients if self.bool_bh_correction else None
+            plan.weight_filter_pack(sgm, packed, self.recon_kernel(), self.det_elem_width,
+                                    self.filter_method, 0, bh)
+            self.last_filtered = None
+        else:
+            filt = self.weight_and_filter(sgm)
+            self.last_filtered = filt
+            packed = ctx.pack(filt, self.storage, layout)
         if out is None:
             out = torch.empty((self.img_dim_z, self.img_dim, self.img_dim), dtype=torch.float32,
                               device=sgm.device)

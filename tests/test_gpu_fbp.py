@@ -126,6 +126,50 @@ def test_weight_filter_fused_equals_separate(ctx, case):
     assert np.count_nonzero(host(got)) > got.numel() // 2
 
 
+@pytest.mark.parametrize("case", ["cone_bf16x3", "cone_wide_bh_bf16", "cone_block_curved"])
+def test_weight_filter_pack_equals_filter_then_pack(ctx, case):
+    """mct_weight_filter_pack = mct_weight_filter_rows then mct_pack_sinogram
+    (FP32, ROWS), bit for bit: the filter's epilogue writes each element with
+    its right neighbour's difference, the columns at the end of a
+    wavefront's 64 are completed afterwards, and the zero rows either side are
+    written.  Row lengths with one (80) and several (200, 1024) such columns,
+    a partial last row tile, BH coefficients and a block of views."""
+    from dataclasses import replace
+
+    bh, view_begin, count = None, 0, None
+    if case == "cone_bf16x3":
+        spec, method, w = cone_case(nu=80, nrows=29), "bf16x3", 0.6
+    elif case == "cone_wide_bh_bf16":
+        spec, method, w = cone_case(nu=1024, nrows=7, nv=40), "bf16", 0.4
+        bh = [1.0, 0.05, -0.002]
+    else:
+        spec, method, w = replace(cone_case(nu=200, nrows=13, nv=90), curved=True), "bf16x3", 0.5
+        view_begin, count = 23, 41
+    count = count or spec.n_views
+    rng = np.random.default_rng(5)
+    raw = rng.uniform(0, 2, (spec.n_v, count, spec.n_u)).astype(np.float32)
+    taps = recon.ramlak_taps(spec.n_u, w, np.float32)
+    plan = Plan(ctx, spec)
+    sgm = dev(raw)
+    filt = torch.empty_like(sgm)
+    plan.weight_filter(sgm, filt, taps, w, method, view_begin, bh)
+    want = ctx.pack(filt, "f32", "rows")
+    got = torch.full_like(want, 0x5A)  # poisoned: every byte must be written
+    plan.weight_filter_pack(sgm, got, taps, w, method, view_begin, bh)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    assert torch.equal(sgm.cpu(), torch.from_numpy(raw))
+
+
+def test_weight_filter_pack_refuses_row_lengths_it_cannot_take(ctx):
+    spec = cone_case(nu=65)  # n_u - 1 = 64: the last element's neighbour has no column
+    plan = Plan(ctx, spec)
+    sgm = torch.zeros((spec.n_v, spec.n_views, spec.n_u), device=DEV)
+    packed = ctx.pack(sgm, "f32", "rows")
+    with pytest.raises(RuntimeError, match="multiple of 64"):
+        plan.weight_filter_pack(sgm, packed, recon.ramlak_taps(spec.n_u, 0.6, np.float32), 0.6, "bf16x3")
+
+
 def test_weight_filter_fused_rejects_the_f32_method(ctx):
     spec = fan_case()
     plan = Plan(ctx, spec)
