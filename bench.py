@@ -112,7 +112,7 @@ def c1_config(storage: str, filter_method: str) -> dict:
         "DetectorOffsetHorizontal": 0.0, "DetectorElementCountVertical": 256,
         "Views": 1080, "TotalScanAngle": 360.0,
         "SourceIsocenterDistance": 750.0, "SourceDetectorDistance": 1250.0,
-        "ImageDimension": 512, "PixelSize": 0.3, "HammingFilter": 1.0,
+        "ImageDimension": 512, "PixelSize": 0.3, "SheppLoganFilter": True,
         "ConeBeam": True, "DetectorElementHeight": 0.4, "ImageDimensionZ": 256,
         "VoxelHeight": 0.3, "DetectorOffsetVertical": 0.0, "ImageCenterZ": 0.0,
         "SinogramStorage": storage, "RampFilterMethod": filter_method,

@@ -732,8 +732,8 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
                 // tile's rows with a cell of margin on either side)
                 const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - (float)(int)wc.r0;
                 const float dt = cv.mag * zstep_t;
-                const int wmax = wc.w > 0 ? wc.w - 1 : 0;
-                const int lc = min(max((int)cv.uf - (int)wc.c0, 0), wmax);
+                // (an unsigned minimum clamps both ends: a negative index wraps high)
+                const uint32_t lc = min((uint32_t)((int)cv.uf - (int)wc.c0), (uint32_t)((int)wc.w - 1));
                 const unsigned char *src =
                     smem + (uint32_t)((j & 1) * cells + lc) * (uint32_t)sizeof(Raw);
                 // reads in batches of 4, all in flight before the first is used
