@@ -10,17 +10,20 @@
 // magnification, the detector u position, the view weight and the range of
 // slices whose projection stays on the detector - is computed once per view,
 // so the slice loop is branch-free: per voxel-view update one FMA for the
-// detector row coordinate, a clamp, a convert, a fract, one address
-// multiply-add, one vector load of the packed taps and four FMAs.  All ZC
-// loads of a view are independent and in flight together.
+// detector row coordinate, a clamp (direct kernel only), a convert, a fract,
+// one address multiply-add, the packed taps and four FMAs.  All ZC loads of a
+// view are independent and in flight together.
 //
-// The filtered sinogram arrives pre-packed as bilinear coefficients, so the
-// taps of one update are a single 16-byte (FP32) or 8-byte (FP16/BF16) load:
-// QUAD layout [view][row+1][col-1] x {a, b-a, c-a, d-c-b+a} for cone beam
-// (cell rows -1 .. rows-1, rows beyond the detector read as zeros),
-// PAIR layout [row][view][col-1] x {a, b-a} for fan beam.  Workgroups are
-// remapped so that each XCD owns a contiguous range of slice chunks: its L2
-// then only holds the detector rows that range projects onto.
+// The filtered sinogram arrives pre-packed, so the taps of one update are one
+// or two vector loads:
+//   ROWS [view][row+2][col-1] x {a, b-a} for cone beam (element rows -1 ..
+//     rows, rows beyond the detector zero): a cell's four coefficients come
+//     from two vertically neighbouring entries;
+//   QUAD [view][row+1][col-1] x {a, b-a, c-a, d-c-b+a} for cone beam (cell
+//     rows -1 .. rows-1): one 16-byte (FP32) or 8-byte (FP16/BF16) load;
+//   PAIR [row][view][col-1] x {a, b-a} for fan beam.
+// Workgroups are remapped so that each XCD owns a contiguous range of slice
+// chunks: its L2 then only holds the detector rows that range projects onto.
 //
 // Mask semantics (run_mgfbp.py:1142-1144, 1170-1173): a voxel whose
 // projection leaves the detector in any view is 0.  The geometry plan turns
@@ -380,8 +383,8 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged cone back-projector (axial view, flat or curved detector, QUAD
-// taps).  A workgroup owns a 16 x 16-column tile and a chunk of 16 slices
+// LDS-staged cone back-projector (axial view, flat or curved detector, ROWS
+// or QUAD taps; the LDS window holds QUAD cells either way).  A workgroup owns a 16 x 16-column tile and a chunk of 16 slices
 // (one column and all 16 slices per thread).  In one view the tile's voxels
 // read packed cells from a small window of the detector; the window of every
 // (tile, chunk, view) is tabulated once per plan from the tile's corners: a

@@ -156,6 +156,9 @@ class Mgfbp_v2(Mgfbp):
             loaded[s].record(copy)
             return blk
 
+        fused = self.fused_filter_pack()
+        taps = self.recon_kernel() if fused else None
+        bh = self.array_bh_coefficients if self.bool_bh_correction else None
         nxt = upload(0)
         filt = None
         for b, (j0, nb) in enumerate(blocks):
@@ -163,9 +166,12 @@ class Mgfbp_v2(Mgfbp):
             if b + 1 < len(blocks):
                 nxt = upload(b + 1)
             compute.wait_event(loaded[b % 2])
-            filt = self.weight_and_filter(blk, j0)
             pk = packed[:pbytes(nb)]
-            ctx.pack_into(filt, self.storage, layout, pk)
+            if fused:  # the MFMA filter writes the block's ROWS taps itself
+                plan.weight_filter_pack(blk, pk, taps, self.det_elem_width, self.filter_method, j0, bh)
+            else:
+                filt = self.weight_and_filter(blk, j0)
+                ctx.pack_into(filt, self.storage, layout, pk)
             plan.backproject_views(pk, self.storage, layout, j0, nb, out,
                                    finish=b == len(blocks) - 1)
             ev = torch.cuda.Event()
@@ -173,7 +179,7 @@ class Mgfbp_v2(Mgfbp):
             consumed[b % 2] = ev
         # Mgfbp_v2 keeps one view on the device: its filtered-sinogram file
         # holds the last view only (run_mgfbp_v2.py:93, 108-109)
-        self.last_filtered = filt[:, -1:, :]
+        self.last_filtered = filt[:, -1:, :] if filt is not None else None
         return out
 
     def MainFunction(self):

@@ -79,9 +79,12 @@ CASES = {
     "cone-coronal-generic": (_cone_cfg, {"ReconViewMode": "coronal", "ImageDimensionZ": 16}, "auto"),
     "fan-shortscan": (_fan_cfg, {"TotalScanAngle": 200.0}, "auto"),
     "fan-f16": (_fan_cfg, {"SinogramStorage": "f16", "RampFilterMethod": "bf16x3"}, "auto"),
+    # weighting, MFMA filter and ROWS taps in one call per block (mct_weight_filter_pack)
+    "cone-bf16x3-fused": (_cone_cfg, {"RampFilterMethod": "bf16x3",
+                                      "BeamHardeningCorrectionCoefficients": [1.0, 0.05]}, "lds"),
 }
 EXPECT = {"cone-lds": "lds", "cone-direct": "direct", "cone-coronal-generic": "generic",
-          "fan-shortscan": "fan", "fan-f16": "fan"}
+          "fan-shortscan": "fan", "fan-f16": "fan", "cone-bf16x3-fused": "lds"}
 
 
 @pytest.mark.parametrize("block", [7, 32])
@@ -92,6 +95,7 @@ def test_streamed_equals_resident(tmp_path, case, block):
     sgm = _sinogram(one, 11)
     res, got, k1, k2 = _resident_and_streamed(one, streamed, sgm)
     assert k1 == k2
+    assert one.fused_filter_pack() == streamed.fused_filter_pack() == (case == "cone-bf16x3-fused")
     if case in EXPECT:
         assert k2 == EXPECT[case]
     assert np.array_equal(got, res), f"{case}: max |diff| {np.abs(got - res).max():.3g}"
