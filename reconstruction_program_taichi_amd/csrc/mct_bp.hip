@@ -187,9 +187,12 @@ template <> struct Taps<MCT_BF16> {
         return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
     }
     __device__ __forceinline__ static Raw quad_of(Pair p0, Pair p1) {
-        const float c = __uint_as_float(p1 << 16) - __uint_as_float(p0 << 16);
-        const float e = __uint_as_float(p1 & 0xffff0000u) - __uint_as_float(p0 & 0xffff0000u);
-        return make_uint2(p0, (uint32_t)mct::f32_to_bf16_rn(c) | ((uint32_t)mct::f32_to_bf16_rn(e) << 16));
+        typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+        typedef float f32x2 __attribute__((ext_vector_type(2)));
+        const f32x2 ce = {__uint_as_float(p1 << 16) - __uint_as_float(p0 << 16),
+                          __uint_as_float(p1 & 0xffff0000u) - __uint_as_float(p0 & 0xffff0000u)};
+        // both differences rounded to nearest even in one v_cvt_pk_bf16_f32
+        return make_uint2(p0, __builtin_bit_cast(uint32_t, __builtin_convertvector(ce, bf16x2)));
     }
 };
 
