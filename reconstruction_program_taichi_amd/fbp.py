@@ -493,8 +493,14 @@ class Mgfbp:
     def tap_layout(self) -> str:
         """Packed tap layout of the back-projector: ROWS (each element with its
         right neighbour's difference, rows padded by a zero row either side)
-        for cone beam, PAIR (the same, row-major over views) for fan beam."""
-        return "rows" if self.cone_beam else "pair"
+        for axial cone beam, QUAD (a cell's four bilinear coefficients) for the
+        P-matrix and coronal/sagittal cone paths, PAIR (the same as ROWS,
+        row-major over views) for fan beam."""
+        if not self.cone_beam:
+            return "pair"
+        # the P-matrix and coronal/sagittal views run the one-thread-per-voxel
+        # kernel, which reads a QUAD cell in one load (ROWS: two)
+        return "quad" if self.bool_apply_pmatrix or self.recon_view_mode != 1 else "rows"
 
     def weight_and_filter(self, sgm: torch.Tensor, view_begin: int | None = None) -> torch.Tensor:
         """BHCorrection + WeightSgm + FilterSinogram (run_mgfbp.py:63-70,

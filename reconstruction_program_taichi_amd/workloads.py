@@ -272,11 +272,11 @@ def pmatrix(args) -> dict:
     plan = Plan(ctx, spec)
     g = torch.Generator(device="cuda").manual_seed(9)
     filt = torch.rand((spec.n_v, spec.n_views, spec.n_u), generator=g, device="cuda")
-    packed = ctx.pack(filt, "f32", "rows")
+    packed = ctx.pack(filt, "f32", "quad")  # the layout the drop-in packs for the P-matrix path
     vol = torch.empty((spec.img_dim_z, spec.img_dim, spec.img_dim), dtype=torch.float32, device="cuda")
-    plan.backproject_fbp(packed, "f32", "rows", vol)
+    plan.backproject_fbp(packed, "f32", "quad", vol)
     kernel = ctx.bp_last_kernel()
-    t = _sync_time(lambda: plan.backproject_fbp(packed, "f32", "rows", vol), max(1, min(args.steps, 3)))
+    t = _sync_time(lambda: plan.backproject_fbp(packed, "f32", "quad", vol), max(1, min(args.steps, 3)))
     U = 512.0 * 512 * 256 * spec.n_views
     return {"workload": f"P-matrix cone back-projection 512x512x256 from {spec.n_views} views x 256 x 1024",
             "kernel": kernel, "ms": round(t * 1e3, 3), "gups": round(U / t / 1e9, 2)}
