@@ -16,6 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libmct.so"
+PROBE = PKG / "issue_rate_probe"
 ARCH = os.environ.get("MCT_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: products are fused into FMAs only where the source says
@@ -59,6 +60,19 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+def build_issue_probe(verbose: bool = True) -> Path:
+    """The VALU issue-rate probe (csrc/probe/issue_rate.hip), a standalone
+    executable beside the library; it measures which instruction forms
+    gfx950 can issue two at a time (the back-projector's roofline uses it)."""
+    src = CSRC / "probe" / "issue_rate.hip"
+    r = subprocess.run([hipcc(), *CFLAGS, str(src), "-o", str(PROBE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr}")
+    if verbose:
+        print(f"built {PROBE}")
+    return PROBE
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
@@ -72,6 +86,7 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(sources)} sources")
+    build_issue_probe(verbose)
     return LIB
 
 

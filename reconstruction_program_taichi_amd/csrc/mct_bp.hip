@@ -393,7 +393,9 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
 // (tile, chunk, view) is tabulated once per plan from the tile's corners: a
 // voxel's detector coordinates are linear-fractional in its position (and the
 // row coordinate linear in z), so their extremes over the tile are taken at
-// the corners, and one cell of margin on each side covers FP32 rounding.  The
+// the corners; a margin of 1/64 of a cell (plus 2^-18 of the coordinate's
+// magnitude) on each side covers the kernel's FP32 rounding, which stays
+// below 1e-3 of a cell at the sizes the plan accepts.  The
 // workgroup copies the window of view j+1 into one LDS buffer while it
 // interpolates view j from the other, so a packed cell crosses the vector
 // memory path once per tile and view instead of once per voxel.  The mask (a
@@ -403,6 +405,8 @@ __global__ __launch_bounds__(256) void bp_cone_axial_kernel(BpArgs a) {
 
 constexpr int kLdsTile = 16;  // columns per side of a workgroup tile (decode_tile)
 constexpr int kLdsZC = 16;    // slices per thread
+constexpr int kViewBatch = 32;  // per-view table entries staged in LDS at a time (a power of 2)
+static_assert((kViewBatch & (kViewBatch - 1)) == 0 && kViewBatch <= 64, "view batch");
 
 struct FastView {
     uint32_t uf;
@@ -528,10 +532,15 @@ __global__ __launch_bounds__(64) void bp_window_kernel(BpArgs a, BpWin *__restri
         }
     }
     const double cmax = a.nu - 2;
-    const double cl = fmin(fmax(floor(umin) - 1.0, 0.0), cmax);
-    const double ch = fmin(fmax(floor(umax) + 1.0, 0.0), cmax);
-    const double rl = floor(tmin) - 1.0;
-    const double rh = floor(tmax) + 1.0;
+    // cell c serves coordinates [c, c + 1): the window holds the cells of
+    // [min - e, max + e], e covering the kernel's FP32 rounding (1/64 of a
+    // cell, plus 2^-18 of the magnitude for very large detectors)
+    const double eu = 1.0 / 64.0 + fmax(fabs(umin), fabs(umax)) * 0x1p-18;
+    const double et = 1.0 / 64.0 + fmax(fabs(tmin), fabs(tmax)) * 0x1p-18;
+    const double cl = fmin(fmax(floor(umin - eu), 0.0), cmax);
+    const double ch = fmin(fmax(floor(umax + eu), 0.0), cmax);
+    const double rl = floor(tmin - et);
+    const double rh = floor(tmax + et);
     const bool fits = !bad && umin <= umax && tmin <= tmax && rl >= -32768.0 && rh <= 32767.0;
     int w = 32767, h = 32767;
     if (fits) {
@@ -693,17 +702,19 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
                     if (soff[q] < lim) buf[tid + 256 * q] = stage[q];
             }
         };
-        // Per-view table entries pass through LDS in batches of 64 views:
-        // s_vt[i] = {cos, sin, dbeta, weight} of view j0 + i, s_win[i] = window
-        // of view j0 + i + 1.  (Read from global memory inside the loop they
-        // would be vector loads queued behind the window prefetch, and
-        // waiting for them would wait for the prefetch.)
-        __shared__ float4 s_vt[64];
-        __shared__ BpWin s_win[64];
+        // Per-view table entries pass through LDS in batches of kViewBatch
+        // views: s_vt[i] = {cos, sin, dbeta, weight} of view j0 + i, s_win[i]
+        // = window of view j0 + i + 1.  (Read from global memory inside the
+        // loop they would be vector loads queued behind the window prefetch,
+        // and waiting for them would wait for the prefetch.)  Batches of 32
+        // rather than 64 keep the static LDS at 1 KB, which with 12.8 KB
+        // windows lets 6 workgroups share a CU instead of 5.
+        __shared__ float4 s_vt[kViewBatch];
+        __shared__ BpWin s_win[kViewBatch];
         auto load_tables = [&](int j0) {
-            if (tid < 64) {
+            if (tid < kViewBatch) {
                 if (j0 + tid < nvb) s_vt[tid] = vt[j0 + tid];
-            } else if (tid < 128) {
+            } else if (tid >= 64 && tid < 64 + kViewBatch) {
                 const int jj = j0 + 1 + (tid - 64);
                 if (jj < nvb) s_win[tid - 64] = wt[jj];
             }
@@ -721,21 +732,21 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
         put(0, wc);
         __syncthreads();
         for (int j = 0; j < nvb; ++j) {
-            if (j && (j & 63) == 0) {  // every wave has finished the previous batch's entries
+            if (j && (j & (kViewBatch - 1)) == 0) {  // every wave has finished the previous batch's entries
                 load_tables(j);
                 __syncthreads();
             }
             const bool more = j + 1 < nvb;
             BpWin wn = wc;
             if (more) {
-                wn = s_win[j & 63];
+                wn = s_win[j & (kViewBatch - 1)];
                 fetch(j + 1, wn);
             }
             if (alive) {
-                const FastView cv = column_view_fast(a, x, y, s_vt[j & 63]);
+                const FastView cv = column_view_fast(a, x, y, s_vt[j & (kViewBatch - 1)]);
                 // row coordinate relative to the window's first row: in
                 // [0, h) for every slice of the chunk (the window spans the
-                // tile's rows with a cell of margin on either side)
+                // tile's rows with a rounding margin on either side)
                 const float t0 = (cv.mag * zfirst - a.v0) * a.vinv - (float)(int)wc.r0;
                 const float dt = cv.mag * zstep_t;
                 // (an unsigned minimum clamps both ends: a negative index wraps high)
@@ -780,8 +791,13 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
     }
 }
 
+// NQ = 2 (the windows of common geometries): at most 80 VGPRs, so 6
+// wavefronts fit a SIMD when the LDS allows 6 workgroups per CU (at
+// 512^2 x 256 / 1080 views: 18.1 ms against 18.7-18.8 ms with 5); wider
+// windows keep the compiler's register choice (80 would spill).
 template <int DT, int NQ, bool ACC, bool ROWS>
-__global__ __launch_bounds__(256) void bp_cone_lds_kernel(BpArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ <= 2 ? 6 : 1)))
+void bp_cone_lds_kernel(BpArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cone_lds_tile<DT, NQ, ACC, ROWS>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
                                smem);

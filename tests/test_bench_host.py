@@ -56,9 +56,17 @@ def test_measured_bounds_arithmetic():
            "per_dispatch": {"FETCH_SIZE_KiB": 1.0e7, "WRITE_SIZE_KiB": 1.0e5, "SQ_INSTS_VALU": 2.4e9,
                             "GRBM_GUI_ACTIVE": 1.2e7},
            "calibration": {"fetch_factor": 2.0, "write_factor": 1.0}}
-    m = bench.measured_bounds(rec, "profiles/x.md", 10.0, 100)
+    rec["per_dispatch"]["SQ_ACTIVE_INST_VALU2"] = 3.0e8
+    m = bench.measured_bounds(rec, "profiles/x.md", 10.0, 100, {"bp_slice_mix": 4.0})
     assert m["hbm_bytes"] == pytest.approx(1.0e7 * 1024 * 2 + 1.0e5 * 1024)
     assert m["hbm_frac"] == pytest.approx(m["hbm_bytes"] / 0.01 / 8e12, rel=1e-3)
-    # 2.4e9 wave instructions over 100 CUs x 2.4 GHz x 10 ms = 1.0
-    assert m["valu_issue_frac"] == pytest.approx(1.0)
-    assert m["valu_issue_frac_busy_cycles"] == pytest.approx(2.4e9 / (100 * 1.2e7))
+    # 2.4e9 wave instructions over 100 CUs and 10 ms: 2.4 per CU per ns, one
+    # per CU per clock at 2.4 GHz, half the two-per-clock dual-issue ceiling
+    assert m["valu_insts_per_cu_per_ns"] == pytest.approx(2.4)
+    assert m["valu_single_issue_frac"] == pytest.approx(1.0)
+    assert m["valu_issue_frac"] == pytest.approx(0.5)
+    assert m["valu_issue_frac_busy_cycles"] == pytest.approx(2.4e9 / (2 * 100 * 1.2e7))
+    # 3e8 dual-issue SIMD cycles over 100 CUs x 1.2e7 clocks (4 SIMDs x 1/4)
+    assert m["valu_dual_issue_cycles_frac"] == pytest.approx(3.0e8 / (100 * 1.2e7))
+    assert m["valu_frac_of_slice_mix_rate"] == pytest.approx(2.4 / 4.0)
+    assert "valu_frac_of_slice_mix_rate" not in bench.measured_bounds(rec, "profiles/x.md", 10.0, 100, None)
