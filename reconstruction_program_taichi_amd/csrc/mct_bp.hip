@@ -36,6 +36,7 @@
 // row coordinate and its two ends are confirmed with the exact test.
 #include "mct_internal.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace {
@@ -76,6 +77,9 @@ struct BpArgs {
     // contributions are added to the volume in view order and masked voxels
     // are marked -10000; 2: as 1, then every voxel below -5000 is set to 0.
     int j0, nvb, accum;
+    // LDS-staged kernel: first 16-slice chunk of the launch (its blocks cover
+    // chunks zc0, zc0 + 1, ...)
+    int zc0;
 };
 
 // Stored value of a voxel whose running sum is acc (alive: it passed the range
@@ -452,7 +456,8 @@ __device__ __forceinline__ float v_coord(const BpArgs &a, float mag, int iz, flo
 // [lo, hi] (lo > hi: none).  d(iz) is non-decreasing in iz (mag > 0, vh > 0),
 // so each view's passing slices form an interval whose ends are found by
 // bisection with the exact test.
-__global__ __launch_bounds__(256) void bp_column_iv_kernel(BpArgs a, int2 *__restrict__ iv) {
+__global__ __launch_bounds__(256) void bp_column_iv_kernel(BpArgs a, int2 *__restrict__ iv,
+                                                           int *__restrict__ kept) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y;
     if (ix >= a.nx) return;
     float x, y;
@@ -488,6 +493,17 @@ __global__ __launch_bounds__(256) void bp_column_iv_kernel(BpArgs a, int2 *__res
         hi = l - 1;
     }
     iv[(size_t)iy * a.nx + ix] = make_int2(lo, hi);
+    // kept[0] = nz - (lowest kept slice), kept[1] = highest kept slice + 1
+    // over all columns (0: none), reduced over the wavefront first
+    int k0 = lo <= hi ? a.nz - lo : 0, k1 = lo <= hi ? hi + 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        k0 = max(k0, __shfl_xor(k0, off));
+        k1 = max(k1, __shfl_xor(k1, off));
+    }
+    if ((threadIdx.x & 63) == 0 && k1 > 0) {
+        atomicMax(&kept[0], k0);
+        atomicMax(&kept[1], k1);
+    }
 }
 
 // Window table, FP64 corner evaluation; maxes[0..1] collect the largest w, h.
@@ -572,9 +588,32 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
     const BpWin *__restrict__ win = a.win;
     const int2 *__restrict__ iv = a.iv;
     const int cells = pitch * rows;
-    int ix, iy, zci;
-    decode_tile(a, block, ix, iy, zci);
-    const int tile = block % (a.tiles_x * a.tiles_y);
+    // Tiles of a chunk in Z order when the tile grid is a square power of
+    // two, so that the workgroups resident at one time form compact blocks
+    // (their detector footprints overlap more than those of a row strip).
+    const int ntile = a.tiles_x * a.tiles_y;
+    const int tz = block % ntile, zci = a.zc0 + block / ntile;
+    if (zci >= a.nzc) return;  // (uniform over the workgroup: before any barrier)
+    int tx, ty;
+    if (a.tiles_x == a.tiles_y && (a.tiles_x & (a.tiles_x - 1)) == 0) {
+        auto even_bits = [](uint32_t v) {
+            v &= 0x55555555u;
+            v = (v | (v >> 1)) & 0x33333333u;
+            v = (v | (v >> 2)) & 0x0f0f0f0fu;
+            v = (v | (v >> 4)) & 0x00ff00ffu;
+            v = (v | (v >> 8)) & 0x0000ffffu;
+            return (int)v;
+        };
+        tx = even_bits((uint32_t)tz);
+        ty = even_bits((uint32_t)tz >> 1);
+    } else {
+        tx = tz % a.tiles_x;
+        ty = tz / a.tiles_x;
+    }
+    const int tile = ty * a.tiles_x + tx;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ix = tx * 16 + (wave & 1) * 8 + (lane & 7);
+    const int iy = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
     const bool inside = ix < a.nx && iy < a.nx;
     const int z0 = zci * kLdsZC;
     const int nk = a.nz - z0 < kLdsZC ? a.nz - z0 : kLdsZC;
@@ -791,6 +830,12 @@ __device__ __forceinline__ void cone_lds_tile(const BpArgs &a, int block, unsign
     }
 }
 
+// Zeroes n floats (the slices a whole-scan launch leaves out).
+__global__ __launch_bounds__(256) void bp_fill_zero_kernel(float *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
 // NQ = 2 (the windows of common geometries): at most 80 VGPRs, so 6
 // wavefronts fit a SIMD when the LDS allows 6 workgroups per CU (at
 // 512^2 x 256 / 1080 views: 18.1 ms against 18.7-18.8 ms with 5); wider
@@ -799,8 +844,8 @@ template <int DT, int NQ, bool ACC, bool ROWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NQ <= 2 ? 6 : 1)))
 void bp_cone_lds_kernel(BpArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cone_lds_tile<DT, NQ, ACC, ROWS>(a, mct::xcd_remap((int)blockIdx.x, a.tiles_x * a.tiles_y * a.nzc, a.nxcd),
-                               smem);
+    // (the grid covers the launch's chunks only: tiles x chunks from zc0)
+    cone_lds_tile<DT, NQ, ACC, ROWS>(a, mct::xcd_remap((int)blockIdx.x, (int)gridDim.x, a.nxcd), smem);
 }
 
 // Fan beam, axial view, PAIR taps; slice z reads detector row z.
@@ -1325,6 +1370,7 @@ BpArgs make_args(const mct_plan *p) {
     a.j0 = 0;
     a.nvb = g.n_views;
     a.accum = 0;
+    a.zc0 = 0;
     return a;
 }
 
@@ -1352,12 +1398,12 @@ int ensure_lds_tables(mct_plan *p, BpArgs a) {
     MCT_HIP(hipMalloc(&p->bp_iv, sizeof(int2) * (size_t)a.nx * a.nx));
     MCT_HIP(hipMalloc(&p->bp_win, sizeof(BpWin) * entries));
     int *maxes = nullptr;
-    MCT_HIP(hipMalloc(&maxes, 2 * sizeof(int)));
-    int host[2] = {0, 0};
-    hipError_t e = hipMemsetAsync(maxes, 0, 2 * sizeof(int), ctx->stream);
+    MCT_HIP(hipMalloc(&maxes, 4 * sizeof(int)));
+    int host[4] = {0, 0, 0, 0};
+    hipError_t e = hipMemsetAsync(maxes, 0, 4 * sizeof(int), ctx->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(bp_column_iv_kernel, dim3(mct::blocks_for(a.nx, 256), a.nx), dim3(256), 0,
-                           ctx->stream, a, p->bp_iv);
+                           ctx->stream, a, p->bp_iv, maxes + 2);
         hipLaunchKernelGGL(bp_window_kernel,
                            dim3(mct::blocks_for(a.nviews, 64), a.tiles_x * a.tiles_y, a.nzc), dim3(64),
                            0, ctx->stream, a, reinterpret_cast<BpWin *>(p->bp_win), maxes);
@@ -1370,6 +1416,8 @@ int ensure_lds_tables(mct_plan *p, BpArgs a) {
         return mct::set_error(MCT_ERR_DEVICE, "back-projector tables: %s", hipGetErrorString(e));
     p->bp_win_w = host[0] > 0 ? host[0] : 1;
     p->bp_win_h = host[1] > 0 ? host[1] : 1;
+    p->bp_zc_lo = host[3] > 0 ? (a.nz - host[2]) / kLdsZC : 0;
+    p->bp_zc_hi = host[3] > 0 ? (host[3] - 1) / kLdsZC : -1;
     if ((int64_t)p->bp_win_w * p->bp_win_h > kLdsMaxCells) {
         hipFree(p->bp_iv);
         hipFree(p->bp_win);
@@ -1434,8 +1482,32 @@ int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
     if (rc) return rc;
     const int cells = a.pitch * a.rows;
     const size_t shmem = lds_bytes(cells, dtype);
-    const int nblocks = a.tiles_x * a.tiles_y * a.nzc;
     mct_ctx *ctx = p->ctx;
+    // A whole-scan launch covers only the chunks where some column keeps a
+    // voxel; the slices of the others are 0, written by a fill.  (Skipping
+    // them also spreads the work evenly over the XCDs: at 512^2 x 256 / 1080
+    // views the first and last of 16 chunks are empty, and the two XCDs that
+    // held one of them sat idle for half the launch.)
+    int nzc_run = a.nzc;
+    if (!a.accum) {
+        const int lo = p->bp_zc_lo, hi = p->bp_zc_hi < a.nzc - 1 ? p->bp_zc_hi : a.nzc - 1;
+        nzc_run = hi >= lo ? hi - lo + 1 : 0;
+        a.zc0 = hi >= lo ? lo : 0;
+        const size_t plane = (size_t)a.nx * a.nx;
+        const int64_t zlo = (int64_t)a.zc0 * kLdsZC;
+        const int64_t zhi = nzc_run ? std::min<int64_t>((int64_t)(a.zc0 + nzc_run) * kLdsZC, a.nz) : 0;
+        if (zlo > 0)
+            hipLaunchKernelGGL(bp_fill_zero_kernel, dim3(mct::blocks_for((int64_t)(zlo * plane), 256)),
+                               dim3(256), 0, ctx->stream, a.vol, (int64_t)(zlo * plane));
+        if (zhi < a.nz)
+            hipLaunchKernelGGL(bp_fill_zero_kernel, dim3(mct::blocks_for((int64_t)((a.nz - zhi) * plane), 256)),
+                               dim3(256), 0, ctx->stream, a.vol + zhi * plane, (int64_t)((a.nz - zhi) * plane));
+        if (nzc_run == 0) {
+            MCT_LAUNCHED();
+            return MCT_OK;
+        }
+    }
+    const int nblocks = a.tiles_x * a.tiles_y * nzc_run;
 #define LDS_LAUNCH2(DT, NQ, ROWS)                                                                 \
     do {                                                                                              \
         if (a.accum)                                                                                  \

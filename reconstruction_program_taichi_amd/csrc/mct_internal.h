@@ -101,6 +101,9 @@ struct mct_plan {
     void *bp_win = nullptr;
     int bp_win_w = 0, bp_win_h = 0;  // largest window over all entries
     int bp_win_state = 0;            // 0 not built, 1 usable, -1 unusable geometry
+    // chunks holding a voxel that some column keeps (every voxel outside
+    // [bp_zc_lo, bp_zc_hi] is 0 in a whole-scan FBP; hi < lo: none)
+    int bp_zc_lo = 0, bp_zc_hi = -1;
 };
 
 // A prepared batched back-projection (mct_bp_batch_create): the job table in
