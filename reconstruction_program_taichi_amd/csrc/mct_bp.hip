@@ -1487,7 +1487,7 @@ int launch_cone_lds(mct_plan *p, BpArgs a, int32_t dtype, bool automatic) {
     // voxel; the slices of the others are 0, written by a fill.  (Skipping
     // them also spreads the work evenly over the XCDs: at 512^2 x 256 / 1080
     // views the first and last of 16 chunks are empty, and the two XCDs that
-    // held one of them sat idle for half the launch.)
+    // held one of them had half the work of the others.)
     int nzc_run = a.nzc;
     if (!a.accum) {
         const int lo = p->bp_zc_lo, hi = p->bp_zc_hi < a.nzc - 1 ? p->bp_zc_hi : a.nzc - 1;
