@@ -265,3 +265,45 @@ def test_kernel_fallbacks_are_reported_once_per_plan():
         quiet[0]._report_fallback_fp()
         quiet[2]._report_fallback_bp("pair")
     assert not [x for x in w if issubclass(x.category, RuntimeWarning)]
+
+
+@pytest.mark.parametrize("n, nz, nv, nu", [(512, 256, 256, 1024), (1024, 1024, 4096, 4096)])
+def test_lds_window_margin_covers_fp32_rounding(n, nz, nv, nu):
+    """The LDS-staged cone back-projector's windows (csrc/mct_bp.hip,
+    bp_window_kernel) hold the cells of [min - e, max + e] of each tile's
+    detector coordinates (FP64, e = 1/64 + 2^-18 |coordinate|), and the slice
+    loop reads its row without a clamp: every voxel's FP32 row coordinate, as
+    the kernel computes it (products and sums rounded one by one, 1/L from a
+    reciprocal within 1 ulp), must stay within e of the exact one.  Checked
+    on random voxels and views of the benchmark geometry and of a 1024^3
+    volume on a 4096 x 4096 detector."""
+    rng = np.random.default_rng(7)
+    f32 = np.float32
+    pix, vh, du, dv, sid, sdd = 0.3, 0.3, 0.4, 0.4, 750.0, 1250.0
+    pix = pix * 512 / n
+    u0, v0 = -(nu - 1) / 2 * du, -(nv - 1) / 2 * dv
+    m = 200_000
+    ix, iy = rng.integers(0, n, m), rng.integers(0, n, m)
+    iz, beta = rng.integers(0, nz, m), rng.uniform(0, 2 * np.pi, m)
+    x = pix * (ix - (n - 1) / 2)
+    y = -pix * (iy - (n - 1) / 2)
+    z = (iz - (nz - 1) / 2) * vh
+    c, s = np.cos(beta), np.sin(beta)
+    # exact (FP64)
+    L = sid - x * c - y * s
+    t64 = (sdd / L * z - v0) / dv
+    u64 = (sdd / L * (-x * s + y * c) - u0) / du
+    # the kernel's FP32 order; the reciprocal off by up to one ulp either way
+    xf, yf, cf, sf = f32(x), f32(y), f32(c), f32(s)
+    Lf = f32(f32(f32(sid) - f32(xf * cf)) - f32(yf * sf))
+    rL = f32(f32(1.0) / Lf)
+    rL = np.where(rng.integers(0, 2, m) == 1, np.nextafter(rL, f32(np.inf)), np.nextafter(rL, f32(0)))
+    mag = f32(f32(sdd) * rL)
+    zf = f32(f32(f32(iz) - f32((nz - 1) / 2)) * f32(vh))
+    t32 = f32(f32(f32(mag * zf) - f32(v0)) * f32(1.0 / dv))
+    yr = f32(f32(f32(-xf) * sf) + f32(yf * cf))
+    u32 = f32(f32(f32(mag * yr) - f32(u0)) * f32(1.0 / du))
+    et = 1.0 / 64 + np.abs(t64) * 2.0 ** -18
+    eu = 1.0 / 64 + np.abs(u64) * 2.0 ** -18
+    assert np.max(np.abs(t32 - t64) / et) < 0.1
+    assert np.max(np.abs(u32 - u64) / eu) < 0.1
